@@ -1,0 +1,210 @@
+/*
+ * lbm_d2q9.h -- C ABI of the MI355X (gfx950) D2Q9 lattice-Boltzmann core.
+ *
+ * Library: lbm_amd/liblbm_d2q9.so (hand-written HIP, built by
+ * lbm_amd/csrc/Makefile).  Every entry point returns an int status
+ * (LBM_OK == 0); lbm_last_error() gives the message of the last failure on
+ * the calling thread.  Pointers named *_dev are device pointers (HBM);
+ * pointers named *_host are host pointers.  `stream` is a hipStream_t passed
+ * as void* (NULL = the default stream).  Nothing in this header depends on
+ * torch: a C, ctypes or cgo caller binds it directly.
+ *
+ * Data layout (same as the reference, lbm/src/core/lattice.py:155-174):
+ *   populations g, g_eq, g_up : double[9][nx][ny], y fastest
+ *   density rho               : double[nx][ny]
+ *   velocity u                : double[2][nx][ny]
+ *   wall inputs u_left/u_right: double[2][ny]; u_top/u_bot: double[2][nx];
+ *   rho_right                 : double[ny]
+ *   obstacle links "boundary" : int64[nb][3] = (i, j, q), q pointing from the
+ *                               fluid node towards the solid (lattice.py:328-344)
+ * The velocity set c, the weights w and the bounce-back table ns are the
+ * fixed D2Q9 tables of lattice.py:135-152; they are not arguments.
+ */
+#ifndef LBM_D2Q9_H
+#define LBM_D2Q9_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_OK              0
+#define LBM_ERR_ARG        -1   /* bad size, null pointer, index out of range */
+#define LBM_ERR_HIP        -2   /* a HIP runtime call failed */
+#define LBM_ERR_UNSUPPORTED -3  /* a boundary-condition plan the fused path cannot express */
+
+#define LBM_ABI_VERSION     1
+
+/* Population storage of the fused path. */
+#define LBM_STORE_F64   0   /* double storage, double arithmetic */
+#define LBM_STORE_F32   1   /* float storage, float arithmetic */
+#define LBM_STORE_F16   2   /* IEEE half storage of f_q - w_q, float arithmetic */
+#define LBM_STORE_BF16  3   /* bfloat16 storage of f_q - w_q, float arithmetic */
+
+/* Wall kinds of the fused boundary-condition plan. */
+#define LBM_BC_NONE      0
+#define LBM_BC_VELOCITY  1   /* nb_zou_he_*_wall_velocity */
+#define LBM_BC_PRESSURE  2   /* nb_zou_he_right_wall_pressure (right wall only) */
+
+/* Sides, in the order of the inlet table rows. */
+#define LBM_SIDE_BOTTOM     0   /* u_bot    [2][nx] */
+#define LBM_SIDE_LEFT       1   /* u_left   [2][ny] */
+#define LBM_SIDE_TOP        2   /* u_top    [2][nx] */
+#define LBM_SIDE_RIGHT      3   /* u_right  [2][ny] */
+#define LBM_SIDE_RHO_RIGHT  4   /* rho_right   [ny] */
+
+int         lbm_abi_version(void);
+const char *lbm_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Operation-by-operation kernels.  Each replaces one reference function with
+ * the same arguments and in-place semantics (double precision, strict
+ * evaluation order, bit-identical to a strict-IEEE evaluation of the
+ * reference expression).  lx, ly of the reference are nx-1, ny-1.
+ * ---------------------------------------------------------------------- */
+
+/* lattice.macro, lbm/src/core/lattice.py:178-189 */
+int lbm_macro(int nx, int ny, const double *g_dev, double *rho_dev, double *u_dev,
+              void *stream);
+
+/* nb_equilibrium(u, c, w, rho, g_eq), lbm/src/core/nb.py:7-17 */
+int lbm_equilibrium(int nx, int ny, const double *u_dev, const double *rho_dev,
+                    double *g_eq_dev, void *stream);
+
+/* nb_col_str(g, g_eq, g_up, om_p, om_m, c, ns, nx, ny, lx, ly), nb.py:21-45 */
+int lbm_col_str(int nx, int ny, double *g_dev, const double *g_eq_dev, double *g_up_dev,
+                double om_p, double om_m, void *stream);
+
+/* nb_drag_lift(boundary, ns, c, g_up, g, R_ref, U_ref, L_ref), nb.py:49-73.
+ * Writes the momentum-exchange sums (fx, fy), accumulated over the links in
+ * order, to out2_dev[0..1]; Cx = -2 fx / (R_ref L_ref U_ref^2) and Cy likewise
+ * (nb.py:70-71) are formed by the caller. */
+int lbm_drag_lift(int nx, int ny, int nb, const int64_t *boundary_dev,
+                  const double *g_up_dev, const double *g_dev, double *out2_dev,
+                  void *stream);
+
+/* nb_bounce_back_obstacle(IBB, boundary, ns, c, ibb, g_up, g, u, lattice),
+ * nb.py:77-117 (the unused u and lattice arguments are dropped). */
+int lbm_bounce_back_obstacle(int nx, int ny, int ibb_on, int nb,
+                             const int64_t *boundary_dev, const double *ibb_dev,
+                             const double *g_up_dev, double *g_dev, void *stream);
+
+/* nb_zou_he_left_wall_velocity(lx, ly, u, u_left, rho, g), nb.py:121-143 */
+int lbm_zou_he_left_wall_velocity(int nx, int ny, double *u_dev, const double *u_left_dev,
+                                  double *rho_dev, double *g_dev, void *stream);
+/* nb_zou_he_right_wall_velocity, nb.py:147-169 */
+int lbm_zou_he_right_wall_velocity(int nx, int ny, double *u_dev, const double *u_right_dev,
+                                   double *rho_dev, double *g_dev, void *stream);
+/* nb_zou_he_right_wall_pressure(lx, ly, u, rho_right, u_right, rho, g), nb.py:173-195 */
+int lbm_zou_he_right_wall_pressure(int nx, int ny, double *u_dev, const double *rho_right_dev,
+                                   const double *u_right_dev, double *rho_dev, double *g_dev,
+                                   void *stream);
+/* nb_zou_he_top_wall_velocity, nb.py:199-221 */
+int lbm_zou_he_top_wall_velocity(int nx, int ny, double *u_dev, const double *u_top_dev,
+                                 double *rho_dev, double *g_dev, void *stream);
+/* nb_zou_he_bottom_wall_velocity, nb.py:225-247 */
+int lbm_zou_he_bottom_wall_velocity(int nx, int ny, double *u_dev, const double *u_bot_dev,
+                                    double *rho_dev, double *g_dev, void *stream);
+/* nb_zou_he_bottom_left_corner_velocity, nb.py:251-271 */
+int lbm_zou_he_bottom_left_corner(int nx, int ny, double *u_dev, double *rho_dev,
+                                  double *g_dev, void *stream);
+/* nb_zou_he_top_left_corner_velocity, nb.py:275-296 */
+int lbm_zou_he_top_left_corner(int nx, int ny, double *u_dev, double *rho_dev,
+                               double *g_dev, void *stream);
+/* nb_zou_he_top_right_corner_velocity, nb.py:300-320 */
+int lbm_zou_he_top_right_corner(int nx, int ny, double *u_dev, double *rho_dev,
+                                double *g_dev, void *stream);
+/* nb_zou_he_bottom_right_corner_velocity, nb.py:324-344 */
+int lbm_zou_he_bottom_right_corner(int nx, int ny, double *u_dev, double *rho_dev,
+                                   double *g_dev, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Fused time step.  One launch of lbm_step performs, for every node, what
+ * the reference does across one pass of its loop (run.py:24-54) between two
+ * macro() calls: pull streaming of the previous post-collision populations,
+ * the recorded boundary-condition plan (obstacle BB/IBB links, Zou-He walls,
+ * Zou-He corners, in the reference's set_bc order), the drag/lift
+ * momentum-exchange terms of the step just finished, the moments, the
+ * equilibrium and the TRT collision.  The state between launches is the
+ * post-collision field g_up, held by the handle in two ping-pong buffers in
+ * the chosen storage format.
+ *
+ * Step numbering: iteration `it` of the reference loop is executed by
+ * lbm_init (it == 0) or lbm_step (it >= 1) for its macro/equilibrium/
+ * collision half, and by the next lbm_step (or lbm_epilogue) for its
+ * stream/boundary/drag half.  Inlet values of iteration it and the drag of
+ * iteration it live in row (it mod ring) of the inlet table and drag ring.
+ * ---------------------------------------------------------------------- */
+typedef struct lbm_fused lbm_fused;
+
+typedef struct {
+    int32_t nx, ny;
+    int32_t storage;            /* LBM_STORE_* */
+    double  om_p, om_m;         /* TRT rates 1/tau_p, 1/tau_m (lattice.py:127-131) */
+    int32_t bc[4];              /* LBM_BC_* for bottom, left, top, right */
+    int32_t corners;            /* 1: the four Zou-He corners follow the walls */
+    int32_t obstacles_first;    /* 1: obstacle links are applied before the walls */
+    int32_t n_links;            /* all links of all obstacles, in plan order */
+    const int64_t *links_host;  /* [n_links][3] (i, j, q); negative i, j wrap */
+    const double  *ibb_host;    /* [n_links] IBB distance p (ignored for plain BB) */
+    const uint8_t *link_ibb_host; /* [n_links] 1 = interpolated (IBB), 0 = plain BB */
+    int32_t n_obs;              /* obstacles (drag/lift sums are kept per obstacle) */
+    const int32_t *obs_offsets_host; /* [n_obs+1] link range of each obstacle */
+    int32_t ring;               /* rows of the inlet table and of the drag ring */
+    int32_t cells_per_thread;   /* 1, 2, 4 or 8 (0 = library default) */
+} lbm_fused_desc;
+
+int  lbm_fused_create(const lbm_fused_desc *desc, lbm_fused **out);
+void lbm_fused_destroy(lbm_fused *h);
+
+/* Inlet table.  Row r holds, for each of the five sides, an offset (in
+ * doubles) into a pool of side arrays.  lbm_fused_set_inlets replaces the
+ * pool (pool_len doubles) and writes nrows rows starting at row0 (mod ring).
+ * Offsets of unused sides may be anything within the pool. */
+int lbm_fused_set_inlets(lbm_fused *h, int row0, int nrows, const int32_t *offsets_host,
+                         int pool_len, const double *pool_host, void *stream);
+
+/* Iteration 0: load g (double[9][nx][ny] on the device), take moments,
+ * collide, store the post-collision field.  When rho_dev/u_dev are not
+ * NULL the moments of g (what lattice.macro() leaves) are stored there. */
+int lbm_init(lbm_fused *h, const double *g_dev, double *rho_dev, double *u_dev, void *stream);
+
+/* Iteration it >= 1 (it is the absolute iteration number). */
+int lbm_step(lbm_fused *h, int64_t it, double *rho_dev, double *u_dev, void *stream);
+
+/* Iterations it0 .. it0+n-1 (it0 >= 1) without moment output, replayed from
+ * a captured HIP graph. */
+int lbm_steps(lbm_fused *h, int64_t it0, int n, void *stream);
+
+/* Stream/boundary/drag half of iteration it (the last one executed):
+ * writes the post-boundary populations g (double[9][nx][ny]), the
+ * post-collision field g_up of iteration it (double), and overwrites rho/u on
+ * the domain perimeter with the values the Zou-He routines leave there. */
+int lbm_epilogue(lbm_fused *h, int64_t it, double *g_dev, double *g_up_dev,
+                 double *rho_dev, double *u_dev, void *stream);
+
+/* Drag/lift terms of iteration it only (link nodes only, no state change). */
+int lbm_drag_only(lbm_fused *h, int64_t it, void *stream);
+
+/* Momentum-exchange sums of iterations it0 .. it0+n-1: out_host[n][n_obs][2]
+ * = (fx, fy), each summed over the obstacle's links in plan order.
+ * Synchronises the stream. */
+int lbm_drag_read(lbm_fused *h, int64_t it0, int n, double *out_host, void *stream);
+
+/* Checkpoint of the post-collision state of iteration it (for rollback). */
+int lbm_fused_save(lbm_fused *h, int64_t it, void *stream);
+int lbm_fused_restore(lbm_fused *h, int64_t it, void *stream);
+
+/* FP64 density-drift check: total mass sum_x sum_q g_up_q(x) of the current
+ * post-collision state of iteration it, reduced in double on the device.
+ * Synchronises the stream. */
+int lbm_mass(lbm_fused *h, int64_t it, double *out_host, void *stream);
+
+/* Bytes of device memory held by the handle. */
+int64_t lbm_fused_device_bytes(const lbm_fused *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_D2Q9_H */

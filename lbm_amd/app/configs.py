@@ -1,0 +1,136 @@
+"""Benchmark configurations of this package, as apps.
+
+* ``cavity200``  -- lid-driven cavity 200x200, u_lid = 0.1, Re = 100,
+  20 001 iterations (it_max = 20000).
+* ``turek200``   -- Schaefer-Turek channel ny = 200 (nx = 1073, D = 48 cells),
+  u_max = 0.05, Re = 100, IBB cylinder, Cd/Cl every iteration, 200 000
+  iterations (stop on the iteration count rather than on convergence).
+* ``obstacle_array`` -- channel 8192x2048 with 64 axis-aligned squares of side
+  32..96 cells at seeded non-overlapping positions (numpy default_rng(seed)),
+  uniform Zou-He inlet u = 0.04 (ramped with sigma = 1000 iterations),
+  Zou-He zero-pressure outlet, no-slip top/bottom, Re = 2000 on the channel
+  height, IBB squares, 10 000 iterations.
+"""
+import math
+
+import numpy as np
+
+from lbm_amd.app._channel import channel_app
+from lbm_amd.app.cavity import cavity
+from lbm_amd.app.turek import turek
+from lbm_amd.obstacle import obstacle
+
+
+class cavity200(cavity):
+    def __init__(self, precision='f64'):
+        super().__init__()
+        self.name = 'cavity200'
+        self.L_lbm = 200
+        self.u_lbm = 0.1
+        self.compute_lbm_parameters()
+        self.it_max = 20000
+        self.precision = precision
+
+
+class turek200(turek):
+    def __init__(self, precision='f64', iterations=200000):
+        super().__init__()
+        self.name = 'turek200'
+        self.L_lbm = 200
+        self.compute_lbm_parameters()
+        self.obstacles[0].size = self.r_cyl
+        self.stop = 'it'
+        self.it_max = iterations - 1
+        self.precision = precision
+
+    def printings(self, it):
+        if getattr(self, "verbose", True) and it % 1000 == 0:
+            print('# it = ' + str(it) + ' / ' + str(self.it_max), end='\r')
+
+
+def place_squares(nx, ny, n_obs, seed, side_lo=32, side_hi=96, margin=4, x_lo=None, x_hi=None):
+    """Seeded non-overlapping axis-aligned squares: list of (cx, cy, side) in
+    lattice nodes, at least ``margin`` free nodes from each other and from the
+    walls."""
+    rng = np.random.default_rng(seed)
+    x_lo = nx // 32 if x_lo is None else x_lo
+    x_hi = nx - nx // 16 if x_hi is None else x_hi
+    placed = []
+    for _ in range(n_obs):
+        side = int(rng.integers(side_lo, side_hi + 1))
+        for _try in range(100000):
+            h = side / 2.0
+            cx = float(rng.uniform(x_lo + h + margin, x_hi - h - margin))
+            cy = float(rng.uniform(h + margin + 1, ny - 1 - h - margin))
+            ok = True
+            for (px, py, ps) in placed:
+                if (abs(cx - px) < h + ps / 2.0 + margin + 2 and
+                        abs(cy - py) < h + ps / 2.0 + margin + 2):
+                    ok = False
+                    break
+            if ok:
+                placed.append((cx, cy, side))
+                break
+        else:
+            raise RuntimeError("could not place all squares")
+    return placed
+
+
+class obstacle_array(channel_app):
+    def __init__(self, ny=2048, aspect=4, n_obs=64, seed=0, precision='f16', iterations=10000,
+                 sigma=1000, n_spts=400):
+        self.name = 'obstacle_array'
+        self.Re_lbm = 2000.0
+        self.L_lbm = ny
+        self.u_lbm = 0.04
+        self.rho_lbm = 1.0
+        self.x_min = 0.0
+        self.x_max = 2.0 * aspect
+        self.y_min = 0.0
+        self.y_max = 2.0
+        self.IBB = True
+        self.stop = 'it'
+        self.output_freq = 1000000000
+        self.output_it = 0
+        self.dpi = 100
+        self.precision = precision
+        self.write_png = False
+        self.Cs = 1.0 / math.sqrt(3.0)
+        self.ny = ny
+        self.nu_lbm = self.u_lbm * self.L_lbm / self.Re_lbm
+        self.tau_lbm = 0.5 + self.nu_lbm / (self.Cs ** 2)
+        self.dx = (self.y_max - self.y_min) / self.ny
+        self.dy = self.dx
+        self.dt = self.Re_lbm * self.nu_lbm / self.L_lbm ** 2
+        self.nx = math.floor(self.ny * (self.x_max - self.x_min) / (self.y_max - self.y_min))
+        self.it_max = iterations - 1
+        self.t_max = self.it_max * self.dt
+        self.sigma = sigma
+        gx = (self.x_max - self.x_min) / (self.nx - 1)
+        gy = (self.y_max - self.y_min) / (self.ny - 1)
+        self.squares = place_squares(self.nx, self.ny, n_obs, seed)
+        self.obstacles = []
+        for k, (cx, cy, side) in enumerate(self.squares):
+            pos = [self.x_min + cx * gx, self.y_min + cy * gy]
+            self.obstacles.append(obstacle('square%d' % k, 4, n_spts, 'square', side * gy, pos))
+        self.drag = []
+
+    def _inlet_profile(self, lattice):
+        prof = getattr(self, "_prof", None)
+        if prof is None:
+            prof = np.zeros((2, lattice.ny))
+            prof[0, :] = 1.0
+            self._prof = prof
+        return prof
+
+    def initialize(self, lattice):
+        self.init_obstacle_fields(lattice)
+        lattice.equilibrium()
+        lattice.g = lattice.g_eq.copy()
+
+    def observables(self, lattice, it):
+        self.drag.append([lattice.drag_lift(o, self.rho_lbm, self.u_lbm, o.size / self.dx)
+                          for o in self.obstacles])
+
+    def printings(self, it):
+        pass

@@ -1,0 +1,1090 @@
+// d2q9_fused.hip -- fused pull-scheme D2Q9 time step for gfx950 (MI355X).
+//
+// One launch per time step.  Each work-item owns a node and
+//   1. pulls its nine populations from the previous post-collision field
+//      (streaming, lbm/src/core/nb.py:37-45, read as a gather),
+//   2. on obstacle-link nodes, replaces the populations the reference's
+//      nb_bounce_back_obstacle writes (nb.py:77-117),
+//   3. on perimeter nodes, applies the Zou-He wall or corner routine
+//      (nb.py:121-344) in registers,
+//   4. on link nodes, stores the momentum-exchange term of nb_drag_lift
+//      (nb.py:64) into a ring row, summed later in link order,
+//   5. takes moments (lattice.py:178-189), forms the equilibrium
+//      (nb.py:7-17) and collides (nb.py:24-35), then stores the nine
+//      post-collision populations.
+// Interior nodes of tiles without links take a branch-free path: nine
+// coalesced loads at constant offsets, arithmetic, nine coalesced stores.
+// HBM traffic per node and step is 9 loads + 9 stores of the storage type.
+//
+// Storage forms: double; float; IEEE half or bfloat16 holding f_q - w_q
+// (float arithmetic in both).  Drag terms are accumulated in double.
+#include "d2q9_common.h"
+#include "../../include/lbm_d2q9.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+namespace lbm {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+// ---------------------------------------------------------------- storage --
+template <int S> struct Store;
+
+template <> struct Store<LBM_STORE_F64> {
+    using s_t = double;
+    using T = double;
+    static __device__ __forceinline__ T ld(const s_t *p, size_t i, int) { return p[i]; }
+    static __device__ __forceinline__ void st(s_t *p, size_t i, int, T v) { p[i] = v; }
+};
+
+template <> struct Store<LBM_STORE_F32> {
+    using s_t = float;
+    using T = float;
+    static __device__ __forceinline__ T ld(const s_t *p, size_t i, int) { return p[i]; }
+    static __device__ __forceinline__ void st(s_t *p, size_t i, int, T v) { p[i] = v; }
+};
+
+template <> struct Store<LBM_STORE_F16> {
+    using s_t = _Float16;
+    using T = float;
+    static __device__ __forceinline__ T ld(const s_t *p, size_t i, int q)
+    {
+        return weight<float>(q) + (float)p[i];
+    }
+    static __device__ __forceinline__ void st(s_t *p, size_t i, int q, T v)
+    {
+        p[i] = (_Float16)(v - weight<float>(q));
+    }
+};
+
+template <> struct Store<LBM_STORE_BF16> {
+    using s_t = uint16_t;
+    using T = float;
+    static __device__ __forceinline__ T ld(const s_t *p, size_t i, int q)
+    {
+        return weight<float>(q) + __uint_as_float((uint32_t)p[i] << 16);
+    }
+    static __device__ __forceinline__ void st(s_t *p, size_t i, int q, T v)
+    {
+        uint32_t u = __float_as_uint(v - weight<float>(q));
+        u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+        p[i] = (uint16_t)(u >> 16);
+    }
+};
+
+// ------------------------------------------------------------------ links --
+// One obstacle link (fluid node `cell`, direction q towards the solid), with
+// the reads of its bounce-back value resolved to (plane, node) pairs and the
+// interpolation factors of nb.py:97-104 formed on the host in double.
+struct LinkRec {
+    int32_t cell;          // fluid node, x*ny + y
+    int32_t cellB, cellC;  // nodes of the 2nd and 3rd reads
+    int8_t q, qb, planeB, planeC;
+    int32_t mode;          // 0 plain bounce-back, 1 interpolated
+    int32_t dslot;         // position of the link in plan order (drag row)
+    double c0, c1, c2;     // value = (c0*A + c1*B) + c2*C
+};
+
+struct Params {
+    const void *src;       // post-collision field of iteration it-1 (storage form)
+    void *dst;             // post-collision field of iteration it
+    const double *g0;      // INIT: populations of iteration 0, double[9][N]
+    double *g_out;         // EPILOGUE: post-boundary populations, double[9][N]
+    double *gup_out;       // EPILOGUE: src converted to double[9][N]
+    double *rho_out, *u_out;
+    int nx, ny, N;
+    uint32_t magic;        // floor(2^32 / ny) + 1, for node -> (x, y)
+    double om_p, b0, a1, a2, a3, a4;
+    int bc_bottom, bc_left, bc_top, bc_right, corners;
+    const double *pool;    // inlet side arrays
+    const int *inlet_off;  // [ring][5] offsets into pool
+    int ring;
+    const int *base;       // device int: row of iteration (it0-1) in graph replays
+    int k;                 // row = (*base + k) mod ring  -- row of iteration it-1
+    const int *tile_slot;  // [n_tiles] -1 or slot of the tile's link offsets
+    const int *cell_off;   // [n_slots][tile_cells+1]
+    const LinkRec *links;  // sorted by node
+    double *drag_ring;     // [ring][n_links]
+    int n_links;
+    const int *link_cells; // DRAG: distinct link nodes
+    int n_link_cells;
+    int tile_shift;        // log2(tile_cells)
+};
+
+enum { MODE_INIT = 0, MODE_STEP = 1, MODE_EPILOGUE = 2, MODE_DRAG = 3 };
+
+constexpr double K1 = 2.0 / 3.0, K2 = 1.0 / 6.0, K3 = 1.0 / 2.0;
+
+__device__ __forceinline__ int node_x(const Params &p, int cell)
+{
+    int x = (int)__umulhi((uint32_t)cell, p.magic);
+    if (cell - x * p.ny < 0) --x;  // the estimate is never low, at most one high
+    return x;
+}
+
+// Zou-He walls on a node's populations, nb.py:121-247, same evaluation order.
+template <typename T> __device__ __forceinline__ void zh_left(T *f, T ux, T uy, T &r)
+{
+    r = (f[0] + f[3] + f[4] + T(2.0) * f[2] + T(2.0) * f[6] + T(2.0) * f[7]) / (T(1.0) - ux);
+    f[1] = f[2] + T(K1) * r * ux;
+    f[5] = f[6] - T(K3) * (f[3] - f[4]) + T(K2) * r * ux + T(K3) * r * uy;
+    f[8] = f[7] + T(K3) * (f[3] - f[4]) + T(K2) * r * ux - T(K3) * r * uy;
+}
+template <typename T> __device__ __forceinline__ void zh_right_tail(T *f, T ux, T uy, T r)
+{
+    f[2] = f[1] - T(K1) * r * ux;
+    f[6] = f[5] + T(K3) * (f[3] - f[4]) - T(K2) * r * ux - T(K3) * r * uy;
+    f[7] = f[8] - T(K3) * (f[3] - f[4]) - T(K2) * r * ux + T(K3) * r * uy;
+}
+template <typename T> __device__ __forceinline__ void zh_right_vel(T *f, T ux, T uy, T &r)
+{
+    r = (f[0] + f[3] + f[4] + T(2.0) * f[1] + T(2.0) * f[5] + T(2.0) * f[8]) / (T(1.0) + ux);
+    zh_right_tail(f, ux, uy, r);
+}
+template <typename T> __device__ __forceinline__ void zh_right_pres(T *f, T r, T uy, T &ux)
+{
+    ux = (f[0] + f[3] + f[4] + T(2.0) * f[1] + T(2.0) * f[5] + T(2.0) * f[8]) / r - T(1.0);
+    zh_right_tail(f, ux, uy, r);
+}
+template <typename T> __device__ __forceinline__ void zh_top(T *f, T ux, T uy, T &r)
+{
+    r = (f[0] + f[1] + f[2] + T(2.0) * f[3] + T(2.0) * f[5] + T(2.0) * f[7]) / (T(1.0) + uy);
+    f[4] = f[3] - T(K1) * r * uy;
+    f[8] = f[7] - T(K3) * (f[1] - f[2]) + T(K3) * r * ux - T(K2) * r * uy;
+    f[6] = f[5] + T(K3) * (f[1] - f[2]) - T(K3) * r * ux - T(K2) * r * uy;
+}
+template <typename T> __device__ __forceinline__ void zh_bottom(T *f, T ux, T uy, T &r)
+{
+    r = (f[0] + f[1] + f[2] + T(2.0) * f[4] + T(2.0) * f[6] + T(2.0) * f[8]) / (T(1.0) - uy);
+    f[3] = f[4] + T(K1) * r * uy;
+    f[5] = f[6] - T(K3) * (f[1] - f[2]) + T(K3) * r * ux + T(K2) * r * uy;
+    f[7] = f[8] + T(K3) * (f[1] - f[2]) - T(K3) * r * ux + T(K2) * r * uy;
+}
+
+// Zou-He corners, nb.py:251-344.  which: 0 BL, 1 TL, 2 TR, 3 BR.
+template <typename T> __device__ __forceinline__ void zh_corner(T *f, int which, T r, T ux, T uy)
+{
+    switch (which) {
+    case 0:
+        f[1] = f[2] + T(K1) * r * ux;
+        f[3] = f[4] + T(K1) * r * uy;
+        f[5] = f[6] + T(K2) * r * ux + T(K2) * r * uy;
+        f[7] = T(0.0);
+        f[8] = T(0.0);
+        break;
+    case 1:
+        f[1] = f[2] + T(K1) * r * ux;
+        f[4] = f[3] - T(K1) * r * uy;
+        f[8] = f[7] + T(K2) * r * ux - T(K2) * r * uy;
+        f[5] = T(0.0);
+        f[6] = T(0.0);
+        break;
+    case 2:
+        f[2] = f[1] - T(K1) * r * ux;
+        f[4] = f[3] - T(K1) * r * uy;
+        f[6] = f[5] - T(K2) * r * ux - T(K2) * r * uy;
+        f[7] = T(0.0);
+        f[8] = T(0.0);
+        break;
+    default:
+        f[2] = f[1] - T(K1) * r * ux;
+        f[3] = f[4] + T(K1) * r * uy;
+        f[7] = f[8] - T(K2) * r * ux + T(K2) * r * uy;
+        f[5] = T(0.0);
+        f[6] = T(0.0);
+        break;
+    }
+    T s = r;
+#pragma unroll
+    for (int q = 1; q < 9; ++q) s = s - f[q];
+    f[0] = s;
+}
+
+__device__ __forceinline__ int bc_row(const Params &p)
+{
+    int r = (*p.base + p.k) % p.ring;
+    return r < 0 ? r + p.ring : r;
+}
+
+__device__ __forceinline__ void link_range(const Params &p, int cell, int &lo, int &hi)
+{
+    lo = hi = 0;
+    if (!p.tile_slot) return;
+    const int slot = p.tile_slot[cell >> p.tile_shift];
+    if (slot < 0) return;
+    const int tc = 1 << p.tile_shift;
+    const int *off = p.cell_off + (size_t)slot * (tc + 1) + (cell & (tc - 1));
+    lo = off[0];
+    hi = off[1];
+}
+
+template <int S>
+__device__ __forceinline__ void pull_checked(const Params &p, int x, int y,
+                                             typename Store<S>::T *f)
+{
+    using St = Store<S>;
+    using T = typename St::T;
+    const auto *src = static_cast<const typename St::s_t *>(p.src);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int sx = x - CX[q], sy = y - CY[q];
+        f[q] = (sx >= 0 && sx < p.nx && sy >= 0 && sy < p.ny)
+                   ? St::ld(src, (size_t)q * p.N + (size_t)sx * p.ny + sy, q)
+                   : T(0);
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void apply_links(const Params &p, int cell, int lo, int hi,
+                                            typename Store<S>::T *f)
+{
+    using St = Store<S>;
+    using T = typename St::T;
+    const auto *src = static_cast<const typename St::s_t *>(p.src);
+    for (int k = lo; k < hi; ++k) {
+        const LinkRec L = p.links[k];
+        const T A = St::ld(src, (size_t)L.q * p.N + cell, L.q);
+        T v = A;
+        if (L.mode) {
+            const T B = St::ld(src, (size_t)L.planeB * p.N + L.cellB, L.planeB);
+            const T C = St::ld(src, (size_t)L.planeC * p.N + L.cellC, L.planeC);
+            v = (T(L.c0) * A + T(L.c1) * B) + T(L.c2) * C;
+        }
+        // f[L.qb] = v with a register-resident array
+#pragma unroll
+        for (int q = 1; q < 9; ++q)
+            if (q == L.qb) f[q] = v;
+    }
+}
+
+// Post-boundary populations of a node that is not a domain corner; r, ux, uy
+// receive what the wall routine writes to rho/u (perimeter nodes only).
+template <int S>
+__device__ void gather_edge(const Params &p, int x, int y, int cell, int row,
+                            typename Store<S>::T *f, typename Store<S>::T &r,
+                            typename Store<S>::T &ux, typename Store<S>::T &uy)
+{
+    using T = typename Store<S>::T;
+    pull_checked<S>(p, x, y, f);
+    int lo, hi;
+    link_range(p, cell, lo, hi);
+    apply_links<S>(p, cell, lo, hi, f);
+    const double *io = p.pool;
+    const int *off = p.inlet_off + row * 5;
+    const int lx = p.nx - 1, ly = p.ny - 1;
+    if (y == 0 && p.bc_bottom) {
+        const double *ub = io + off[LBM_SIDE_BOTTOM];
+        ux = T(ub[x]);
+        uy = T(ub[p.nx + x]);
+        zh_bottom(f, ux, uy, r);
+    } else if (y == ly && p.bc_top) {
+        const double *ut = io + off[LBM_SIDE_TOP];
+        ux = T(ut[x]);
+        uy = T(ut[p.nx + x]);
+        zh_top(f, ux, uy, r);
+    } else if (x == 0 && p.bc_left) {
+        const double *ul = io + off[LBM_SIDE_LEFT];
+        ux = T(ul[y]);
+        uy = T(ul[p.ny + y]);
+        zh_left(f, ux, uy, r);
+    } else if (x == lx && p.bc_right == LBM_BC_VELOCITY) {
+        const double *ur = io + off[LBM_SIDE_RIGHT];
+        ux = T(ur[y]);
+        uy = T(ur[p.ny + y]);
+        zh_right_vel(f, ux, uy, r);
+    } else if (x == lx && p.bc_right == LBM_BC_PRESSURE) {
+        const double *ur = io + off[LBM_SIDE_RIGHT];
+        const double *rr = io + off[LBM_SIDE_RHO_RIGHT];
+        r = T(rr[y]);
+        uy = T(ur[p.ny + y]);
+        zh_right_pres(f, r, uy, ux);
+    }
+}
+
+// Post-boundary populations of any node (corners recompute the adjacent
+// top/bottom wall node, whose rho/u the reference corner routines copy).
+template <int S>
+__device__ void gather_bc(const Params &p, int x, int y, int cell, int row,
+                          typename Store<S>::T *f, typename Store<S>::T &r,
+                          typename Store<S>::T &ux, typename Store<S>::T &uy)
+{
+    using T = typename Store<S>::T;
+    const int lx = p.nx - 1, ly = p.ny - 1;
+    const bool xe = (x == 0 || x == lx), ye = (y == 0 || y == ly);
+    if (!(xe && ye) || !p.corners) {
+        gather_edge<S>(p, x, y, cell, row, f, r, ux, uy);
+        return;
+    }
+    // corner: own streamed populations and links, then the corner routine
+    pull_checked<S>(p, x, y, f);
+    int lo, hi;
+    link_range(p, cell, lo, hi);
+    apply_links<S>(p, cell, lo, hi, f);
+    const int nxn = (x == 0) ? 1 : lx - 1;
+    const int which = (x == 0) ? (y == 0 ? 0 : 1) : (y == 0 ? 3 : 2);
+    T fn[9];
+    gather_edge<S>(p, nxn, y, nxn * p.ny + y, row, fn, r, ux, uy);
+    zh_corner(f, which, r, ux, uy);
+}
+
+template <int S>
+__device__ __forceinline__ void store_drag(const Params &p, int cell, int row,
+                                           const typename Store<S>::T *f)
+{
+    using St = Store<S>;
+    const auto *src = static_cast<const typename St::s_t *>(p.src);
+    int lo, hi;
+    link_range(p, cell, lo, hi);
+    for (int k = lo; k < hi; ++k) {
+        const LinkRec L = p.links[k];
+        double fb = 0.0;
+#pragma unroll
+        for (int q = 1; q < 9; ++q)
+            if (q == L.qb) fb = (double)f[q];
+        const double A = (double)St::ld(src, (size_t)L.q * p.N + cell, L.q);
+        p.drag_ring[(size_t)row * p.n_links + L.dslot] = A + fb;
+    }
+}
+
+// Moments, equilibrium, TRT collision and store (lattice.py:178-189,
+// nb.py:7-17, nb.py:24-35).
+template <int S>
+__device__ __forceinline__ void collide_store(const Params &p, int cell,
+                                              typename Store<S>::T *f, bool macro)
+{
+    using St = Store<S>;
+    using T = typename St::T;
+    auto *dst = static_cast<typename St::s_t *>(p.dst);
+    T r = f[0];
+#pragma unroll
+    for (int q = 1; q < 9; ++q) r = r + f[q];
+    const T sx = ((((f[1] - f[2]) + f[5]) - f[6]) - f[7]) + f[8];
+    const T sy = ((((f[3] - f[4]) + f[5]) - f[6]) + f[7]) - f[8];
+    const T ux = sx / r, uy = sy / r;
+    if (macro) {
+        p.rho_out[cell] = (double)r;
+        p.u_out[cell] = (double)ux;
+        p.u_out[(size_t)p.N + cell] = (double)uy;
+    }
+    const T v = T(1.5) * (ux * ux + uy * uy);
+    T e[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const T t = T(3.0) * (ux * T(CX[q]) + uy * T(CY[q]));
+        e[q] = (T(1.0) + t + T(0.5) * (t * t) - v) * (r * weight<T>(q));
+    }
+    St::st(dst, cell, 0, T(p.b0) * f[0] + T(p.om_p) * e[0]);
+    const T a1 = T(p.a1), a2 = T(p.a2), a3 = T(p.a3), a4 = T(p.a4);
+#pragma unroll
+    for (int q = 1; q < 9; ++q) {
+        const int qb = NS[q];
+        St::st(dst, (size_t)q * p.N + cell, q, a1 * f[q] - a2 * f[qb] + a3 * e[q] + a4 * e[qb]);
+    }
+}
+
+// Work-group index -> tile index, so that each XCD (work-groups are dealt
+// round-robin over the 8 XCDs) sweeps one contiguous slab of the lattice and
+// the rows its tiles share stay in that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    const int q = nb >> 3, rr = nb & 7;
+    const int xcd = b & 7, i = b >> 3;
+    return xcd * q + min(xcd, rr) + i;
+}
+
+template <int S, int MODE, int CPT>
+__global__ __launch_bounds__(256) void k_fused(Params p)
+{
+    using St = Store<S>;
+    using T = typename St::T;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int base = tile * (256 * CPT) + threadIdx.x;
+    const bool link_tile = p.tile_slot && p.tile_slot[tile] >= 0;
+    const int lx = p.nx - 1, ly = p.ny - 1;
+    const int row = (MODE == MODE_INIT) ? 0 : bc_row(p);
+    const bool macro = p.rho_out != nullptr;
+#pragma unroll 1
+    for (int c = 0; c < CPT; ++c) {
+        const int cell = base + c * 256;
+        if (cell >= p.N) return;
+        T f[9];
+        if (MODE == MODE_INIT) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) f[q] = T(p.g0[(size_t)q * p.N + cell]);
+            collide_store<S>(p, cell, f, macro);
+            continue;
+        }
+        const int x = node_x(p, cell), y = cell - x * p.ny;
+        const bool interior = (unsigned)(x - 1) < (unsigned)(lx - 1) &&
+                              (unsigned)(y - 1) < (unsigned)(ly - 1);
+        if (MODE == MODE_STEP && interior && !link_tile) {
+            const auto *src = static_cast<const typename St::s_t *>(p.src);
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                f[q] = St::ld(src, q * p.N + cell - (CX[q] * p.ny + CY[q]), q);
+            collide_store<S>(p, cell, f, macro);
+            continue;
+        }
+        T r = T(0), ux = T(0), uy = T(0);
+        gather_bc<S>(p, x, y, cell, row, f, r, ux, uy);
+        if (link_tile) store_drag<S>(p, cell, row, f);
+        if (MODE == MODE_STEP) {
+            collide_store<S>(p, cell, f, macro);
+        } else {  // MODE_EPILOGUE
+            const auto *src = static_cast<const typename St::s_t *>(p.src);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                p.g_out[(size_t)q * p.N + cell] = (double)f[q];
+                p.gup_out[(size_t)q * p.N + cell] = (double)St::ld(src, (size_t)q * p.N + cell, q);
+            }
+            if (!interior && p.rho_out) {
+                p.rho_out[cell] = (double)r;
+                p.u_out[cell] = (double)ux;
+                p.u_out[(size_t)p.N + cell] = (double)uy;
+            }
+        }
+    }
+}
+
+// Drag terms of one iteration on the link nodes only.
+template <int S>
+__global__ __launch_bounds__(256) void k_drag_only(Params p)
+{
+    using T = typename Store<S>::T;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.n_link_cells) return;
+    const int cell = p.link_cells[i];
+    const int x = node_x(p, cell), y = cell - x * p.ny;
+    const int row = bc_row(p);
+    T f[9], r = T(0), ux = T(0), uy = T(0);
+    gather_bc<S>(p, x, y, cell, row, f, r, ux, uy);
+    store_drag<S>(p, cell, row, f);
+}
+
+// Per-obstacle sums of the ring rows, in link plan order (nb.py:57-67).
+__global__ void k_drag_reduce(const double *__restrict__ ringbuf, int n_links, int ring,
+                              int64_t it0, int n, const int *__restrict__ obs_off, int n_obs,
+                              const int8_t *__restrict__ plan_q, double *__restrict__ out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * n_obs) return;
+    const int s = t / n_obs, o = t - s * n_obs;
+    const int row = (int)((it0 + s) % ring);
+    const double *r = ringbuf + (size_t)row * n_links;
+    double fx = 0.0, fy = 0.0;
+    for (int k = obs_off[o]; k < obs_off[o + 1]; ++k) {
+        const double g0 = r[k];
+        fx = fx + g0 * (double)CX[plan_q[k]];
+        fy = fy + g0 * (double)CY[plan_q[k]];
+    }
+    out[2 * (size_t)t] = fx;
+    out[2 * (size_t)t + 1] = fy;
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_mass_partial(const void *buf, int N, double *part)
+{
+    using St = Store<S>;
+    const auto *src = static_cast<const typename St::s_t *>(buf);
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < N; c += gridDim.x * 256) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) s += (double)St::ld(src, (size_t)q * N + c, q);
+        acc += s;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_sum_serial(const double *part, int n, double *out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += part[i];
+    out[0] = s;
+}
+
+__global__ void k_set_int(int *p, int v)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+// ------------------------------------------------------------------ host --
+size_t store_bytes(int s) { return s == LBM_STORE_F64 ? 8 : (s == LBM_STORE_F32 ? 4 : 2); }
+
+struct GraphKey {
+    int n, parity;
+    bool operator<(const GraphKey &o) const { return n != o.n ? n < o.n : parity < o.parity; }
+};
+
+}  // namespace lbm
+
+using namespace lbm;
+
+struct lbm_fused {
+    int nx = 0, ny = 0, N = 0, storage = 0, cpt = 1, tile_cells = 256, tile_shift = 8, n_tiles = 0;
+    uint32_t magic = 0;
+    double om_p = 0, b0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+    int bc[4] = {0, 0, 0, 0};
+    int corners = 0;
+    int n_links = 0, n_obs = 0, ring = 0, n_slots = 0, n_link_cells = 0;
+    void *buf[2] = {nullptr, nullptr};
+    void *ckpt = nullptr;
+    int64_t ckpt_it = -1;
+    int *d_tile_slot = nullptr, *d_cell_off = nullptr, *d_link_cells = nullptr;
+    LinkRec *d_links = nullptr;
+    int *d_obs_off = nullptr;
+    int8_t *d_plan_q = nullptr;
+    double *d_drag_ring = nullptr, *d_drag_out = nullptr;
+    int drag_out_cap = 0;
+    double *d_pool = nullptr;
+    int pool_cap = 0;
+    int *d_inlet_off = nullptr;
+    int *d_base = nullptr, *d_zero = nullptr;
+    double *d_part = nullptr;
+    int n_part = 0;
+    hipStream_t cap_stream = nullptr;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    int64_t bytes = 0;
+};
+
+namespace {
+
+template <typename P> int dmalloc(lbm_fused *h, P **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess)
+        return set_error(LBM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    h->bytes += (int64_t)bytes;
+    return LBM_OK;
+}
+
+Params base_params(const lbm_fused *h)
+{
+    Params p{};
+    p.nx = h->nx;
+    p.ny = h->ny;
+    p.N = h->N;
+    p.magic = h->magic;
+    p.om_p = h->om_p;
+    p.b0 = h->b0;
+    p.a1 = h->a1;
+    p.a2 = h->a2;
+    p.a3 = h->a3;
+    p.a4 = h->a4;
+    p.bc_bottom = h->bc[0];
+    p.bc_left = h->bc[1];
+    p.bc_top = h->bc[2];
+    p.bc_right = h->bc[3];
+    p.corners = h->corners;
+    p.pool = h->d_pool;
+    p.inlet_off = h->d_inlet_off;
+    p.ring = h->ring;
+    p.base = h->d_zero;
+    p.k = 0;
+    p.tile_slot = h->n_slots ? h->d_tile_slot : nullptr;
+    p.cell_off = h->d_cell_off;
+    p.links = h->d_links;
+    p.drag_ring = h->d_drag_ring;
+    p.n_links = h->n_links;
+    p.link_cells = h->d_link_cells;
+    p.n_link_cells = h->n_link_cells;
+    p.tile_shift = h->tile_shift;
+    return p;
+}
+
+int row_of(const lbm_fused *h, int64_t it)
+{
+    int64_t r = it % h->ring;
+    return (int)(r < 0 ? r + h->ring : r);
+}
+
+template <int S, int MODE>
+void launch_fused_s(const lbm_fused *h, const Params &p, hipStream_t s)
+{
+    const int grid = h->n_tiles;
+    switch (h->cpt) {
+    case 1: k_fused<S, MODE, 1><<<grid, 256, 0, s>>>(p); break;
+    case 2: k_fused<S, MODE, 2><<<grid, 256, 0, s>>>(p); break;
+    case 4: k_fused<S, MODE, 4><<<grid, 256, 0, s>>>(p); break;
+    default: k_fused<S, MODE, 8><<<grid, 256, 0, s>>>(p); break;
+    }
+}
+
+template <int MODE> void launch_fused(const lbm_fused *h, const Params &p, hipStream_t s)
+{
+    switch (h->storage) {
+    case LBM_STORE_F64: launch_fused_s<LBM_STORE_F64, MODE>(h, p, s); break;
+    case LBM_STORE_F32: launch_fused_s<LBM_STORE_F32, MODE>(h, p, s); break;
+    case LBM_STORE_F16: launch_fused_s<LBM_STORE_F16, MODE>(h, p, s); break;
+    default: launch_fused_s<LBM_STORE_BF16, MODE>(h, p, s); break;
+    }
+}
+
+void launch_drag_only(const lbm_fused *h, const Params &p, hipStream_t s)
+{
+    const int grid = (h->n_link_cells + 255) / 256;
+    if (grid == 0) return;
+    switch (h->storage) {
+    case LBM_STORE_F64: k_drag_only<LBM_STORE_F64><<<grid, 256, 0, s>>>(p); break;
+    case LBM_STORE_F32: k_drag_only<LBM_STORE_F32><<<grid, 256, 0, s>>>(p); break;
+    case LBM_STORE_F16: k_drag_only<LBM_STORE_F16><<<grid, 256, 0, s>>>(p); break;
+    default: k_drag_only<LBM_STORE_BF16><<<grid, 256, 0, s>>>(p); break;
+    }
+}
+
+inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Builds the node-sorted link records and the per-tile offset tables.
+int build_links(lbm_fused *h, const lbm_fused_desc *d)
+{
+    const int nx = h->nx, ny = h->ny, nl = d->n_links;
+    std::vector<LinkRec> recs(nl);
+    std::vector<int8_t> plan_q(nl);
+    auto wrap = [](long long k, int n, long long &o) {
+        if (k < -(long long)n || k >= n) return false;
+        o = k < 0 ? k + n : k;
+        return true;
+    };
+    for (int k = 0; k < nl; ++k) {
+        const long long ri = d->links_host[3 * k], rj = d->links_host[3 * k + 1];
+        const long long q = d->links_host[3 * k + 2];
+        long long i, j;
+        if (!wrap(ri, nx, i) || !wrap(rj, ny, j) || q < 1 || q > 8)
+            return set_error(LBM_ERR_ARG, "link " + std::to_string(k) + " out of range");
+        LinkRec L{};
+        L.cell = (int)(i * ny + j);
+        L.q = (int8_t)q;
+        L.qb = (int8_t)NS[q];
+        L.dslot = k;
+        L.mode = d->link_ibb_host ? (d->link_ibb_host[k] ? 1 : 0) : 0;
+        plan_q[k] = (int8_t)q;
+        if (L.mode) {
+            const int qb = NS[q];
+            long long im, jm, imm, jmm;
+            // cb = c[qb] (nb.py:89-93); indices wrap like NumPy's
+            if (!wrap(ri + CX[qb], nx, im) || !wrap(rj + CY[qb], ny, jm) ||
+                !wrap(ri + 2 * CX[qb], nx, imm) || !wrap(rj + 2 * CY[qb], ny, jmm))
+                return set_error(LBM_ERR_ARG,
+                                 "IBB stencil of link " + std::to_string(k) + " leaves the lattice");
+            const double p = d->ibb_host[k];
+            const double pp = 2.0 * p;
+            if (p < 0.5) {  // nb.py:97-100
+                L.planeB = (int8_t)q;
+                L.cellB = (int)(im * ny + jm);
+                L.planeC = (int8_t)q;
+                L.cellC = (int)(imm * ny + jmm);
+                L.c0 = p * (pp + 1.0);
+                L.c1 = (1.0 + pp) * (1.0 - pp);
+                L.c2 = -(p * (1.0 - pp));
+            } else {        // nb.py:101-104
+                L.planeB = (int8_t)qb;
+                L.cellB = L.cell;
+                L.planeC = (int8_t)qb;
+                L.cellC = (int)(im * ny + jm);
+                L.c0 = 1.0 / (p * (pp + 1.0));
+                L.c1 = (pp - 1.0) / p;
+                L.c2 = (1.0 - pp) / (1.0 + pp);
+            }
+        }
+        recs[k] = L;
+    }
+    if (!d->obstacles_first) {
+        for (const auto &L : recs) {
+            const int x = L.cell / ny, y = L.cell % ny;
+            if (x == 0 || x == nx - 1 || y == 0 || y == ny - 1)
+                return set_error(LBM_ERR_UNSUPPORTED,
+                                 "obstacle links on the perimeter applied after the walls");
+        }
+    }
+    std::stable_sort(recs.begin(), recs.end(),
+                     [](const LinkRec &a, const LinkRec &b) { return a.cell < b.cell; });
+    std::vector<int> tile_slot(h->n_tiles, -1);
+    std::vector<int> cell_off;
+    std::vector<int> link_cells;
+    const int tc = h->tile_cells;
+    int slot = 0;
+    for (int k = 0; k < nl;) {
+        const int tile = recs[k].cell >> h->tile_shift;
+        tile_slot[tile] = slot;
+        const size_t base = cell_off.size();
+        cell_off.resize(base + tc + 1, 0);
+        int k2 = k;
+        std::vector<int> cnt(tc, 0);
+        while (k2 < nl && (recs[k2].cell >> h->tile_shift) == tile) {
+            ++cnt[recs[k2].cell & (tc - 1)];
+            if (link_cells.empty() || link_cells.back() != recs[k2].cell)
+                link_cells.push_back(recs[k2].cell);
+            ++k2;
+        }
+        int acc = k;
+        for (int c = 0; c < tc; ++c) {
+            cell_off[base + c] = acc;
+            acc += cnt[c];
+        }
+        cell_off[base + tc] = acc;
+        k = k2;
+        ++slot;
+    }
+    h->n_slots = slot;
+    h->n_link_cells = (int)link_cells.size();
+    int e;
+    if ((e = dmalloc(h, &h->d_tile_slot, sizeof(int) * h->n_tiles))) return e;
+    if ((e = dmalloc(h, &h->d_cell_off, sizeof(int) * cell_off.size()))) return e;
+    if ((e = dmalloc(h, &h->d_links, sizeof(LinkRec) * nl))) return e;
+    if ((e = dmalloc(h, &h->d_link_cells, sizeof(int) * link_cells.size()))) return e;
+    if ((e = dmalloc(h, &h->d_plan_q, nl))) return e;
+    LBM_HIP_TRY(hipMemcpy(h->d_tile_slot, tile_slot.data(), sizeof(int) * h->n_tiles,
+                          hipMemcpyHostToDevice));
+    if (!cell_off.empty())
+        LBM_HIP_TRY(hipMemcpy(h->d_cell_off, cell_off.data(), sizeof(int) * cell_off.size(),
+                              hipMemcpyHostToDevice));
+    if (nl) {
+        LBM_HIP_TRY(hipMemcpy(h->d_links, recs.data(), sizeof(LinkRec) * nl, hipMemcpyHostToDevice));
+        LBM_HIP_TRY(hipMemcpy(h->d_plan_q, plan_q.data(), nl, hipMemcpyHostToDevice));
+        LBM_HIP_TRY(hipMemcpy(h->d_link_cells, link_cells.data(), sizeof(int) * link_cells.size(),
+                              hipMemcpyHostToDevice));
+    }
+    // obstacle link ranges
+    std::vector<int> obs_off(d->n_obs + 1, 0);
+    if (d->n_obs > 0) {
+        for (int o = 0; o <= d->n_obs; ++o) obs_off[o] = d->obs_offsets_host[o];
+        if (obs_off[0] != 0 || obs_off[d->n_obs] != nl)
+            return set_error(LBM_ERR_ARG, "obs_offsets must span [0, n_links]");
+        for (int o = 0; o < d->n_obs; ++o)
+            if (obs_off[o + 1] < obs_off[o]) return set_error(LBM_ERR_ARG, "obs_offsets decrease");
+    }
+    if ((e = dmalloc(h, &h->d_obs_off, sizeof(int) * obs_off.size()))) return e;
+    LBM_HIP_TRY(hipMemcpy(h->d_obs_off, obs_off.data(), sizeof(int) * obs_off.size(),
+                          hipMemcpyHostToDevice));
+    return LBM_OK;
+}
+
+void free_all(lbm_fused *h)
+{
+    for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+    h->graphs.clear();
+    void *ptrs[] = {h->buf[0], h->buf[1], h->ckpt, h->d_tile_slot, h->d_cell_off,
+                    h->d_link_cells, h->d_links, h->d_obs_off, h->d_plan_q, h->d_drag_ring,
+                    h->d_drag_out, h->d_pool, h->d_inlet_off, h->d_base, h->d_zero, h->d_part};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
+}
+
+}  // namespace
+
+extern "C" int lbm_abi_version(void) { return LBM_ABI_VERSION; }
+
+extern "C" const char *lbm_last_error(void) { return lbm::g_last_error.c_str(); }
+
+extern "C" int64_t lbm_fused_device_bytes(const lbm_fused *h) { return h ? h->bytes : 0; }
+
+extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
+{
+    if (!d || !out) return set_error(LBM_ERR_ARG, "lbm_fused_create: null argument");
+    *out = nullptr;
+    if (d->nx < 4 || d->ny < 4) return set_error(LBM_ERR_ARG, "nx and ny must be >= 4");
+    if ((long long)d->nx * d->ny * 9 >= (1LL << 31))
+        return set_error(LBM_ERR_ARG, "lattice too large for 32-bit population indices");
+    if (d->storage < 0 || d->storage > 3) return set_error(LBM_ERR_ARG, "bad storage");
+    if (d->ring < 2) return set_error(LBM_ERR_ARG, "ring must be >= 2");
+    if (d->n_links < 0 || (d->n_links > 0 && (!d->links_host || !d->ibb_host)))
+        return set_error(LBM_ERR_ARG, "bad link arrays");
+    if (d->n_obs < 0 || (d->n_obs > 0 && !d->obs_offsets_host))
+        return set_error(LBM_ERR_ARG, "bad obstacle ranges");
+    // The fused plan: the four Zou-He walls (right wall velocity or pressure)
+    // followed by the four corners, as every reference app's set_bc does.
+    if (d->bc[0] != LBM_BC_VELOCITY || d->bc[1] != LBM_BC_VELOCITY ||
+        d->bc[2] != LBM_BC_VELOCITY ||
+        (d->bc[3] != LBM_BC_VELOCITY && d->bc[3] != LBM_BC_PRESSURE) || !d->corners)
+        return set_error(LBM_ERR_UNSUPPORTED,
+                         "fused plan needs Zou-He velocity walls on bottom/left/top, velocity "
+                         "or pressure on the right, and the four corners");
+    auto h = std::make_unique<lbm_fused>();
+    h->nx = d->nx;
+    h->ny = d->ny;
+    h->N = d->nx * d->ny;
+    h->storage = d->storage;
+    h->cpt = d->cells_per_thread;
+    if (h->cpt == 0) h->cpt = 2;
+    if (h->cpt != 1 && h->cpt != 2 && h->cpt != 4 && h->cpt != 8)
+        return set_error(LBM_ERR_ARG, "cells_per_thread must be 1, 2, 4 or 8");
+    h->tile_cells = 256 * h->cpt;
+    h->tile_shift = 8 + (h->cpt == 1 ? 0 : (h->cpt == 2 ? 1 : (h->cpt == 4 ? 2 : 3)));
+    h->n_tiles = (h->N + h->tile_cells - 1) / h->tile_cells;
+    h->magic = (uint32_t)((1ULL << 32) / (uint64_t)d->ny) + 1u;
+    h->om_p = d->om_p;
+    // the scalar factors of nb.py:25 and nb.py:32-35
+    h->b0 = 1.0 - d->om_p;
+    h->a1 = 1.0 - 0.5 * (d->om_p + d->om_m);
+    h->a2 = 0.5 * (d->om_p - d->om_m);
+    h->a3 = 0.5 * (d->om_p + d->om_m);
+    h->a4 = 0.5 * (d->om_p - d->om_m);
+    for (int i = 0; i < 4; ++i) h->bc[i] = d->bc[i];
+    h->corners = d->corners;
+    h->n_links = d->n_links;
+    h->n_obs = d->n_obs;
+    h->ring = d->ring;
+    int e;
+    const size_t sb = store_bytes(d->storage) * 9 * (size_t)h->N;
+    if ((e = dmalloc(h.get(), &h->buf[0], sb)) || (e = dmalloc(h.get(), &h->buf[1], sb))) {
+        free_all(h.get());
+        return e;
+    }
+    if ((e = build_links(h.get(), d))) {
+        free_all(h.get());
+        return e;
+    }
+    if ((e = dmalloc(h.get(), &h->d_drag_ring, sizeof(double) * (size_t)h->ring *
+                                                   (h->n_links ? h->n_links : 1))) ||
+        (e = dmalloc(h.get(), &h->d_inlet_off, sizeof(int) * 5 * (size_t)h->ring)) ||
+        (e = dmalloc(h.get(), &h->d_base, sizeof(int))) ||
+        (e = dmalloc(h.get(), &h->d_zero, sizeof(int)))) {
+        free_all(h.get());
+        return e;
+    }
+    if (hipMemset(h->d_inlet_off, 0, sizeof(int) * 5 * (size_t)h->ring) != hipSuccess ||
+        hipMemset(h->d_zero, 0, sizeof(int)) != hipSuccess ||
+        hipMemset(h->d_base, 0, sizeof(int)) != hipSuccess) {
+        free_all(h.get());
+        return set_error(LBM_ERR_HIP, "hipMemset failed");
+    }
+    if (hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+        free_all(h.get());
+        return set_error(LBM_ERR_HIP, "hipStreamCreate failed");
+    }
+    *out = h.release();
+    return LBM_OK;
+}
+
+extern "C" void lbm_fused_destroy(lbm_fused *h)
+{
+    if (!h) return;
+    (void)hipDeviceSynchronize();
+    free_all(h);
+    delete h;
+}
+
+extern "C" int lbm_fused_set_inlets(lbm_fused *h, int row0, int nrows, const int32_t *off_host,
+                                    int pool_len, const double *pool_host, void *stream)
+{
+    if (!h || nrows < 0 || nrows > h->ring || pool_len < 0 || (nrows && !off_host) ||
+        (pool_len && !pool_host))
+        return set_error(LBM_ERR_ARG, "lbm_fused_set_inlets: bad argument");
+    for (int i = 0; i < 5 * nrows; ++i)
+        if (off_host[i] < 0 || (off_host[i] > 0 && off_host[i] >= pool_len))
+            return set_error(LBM_ERR_ARG, "inlet offset outside the pool");
+    // the previous contents may still be read by queued launches
+    LBM_HIP_TRY(hipStreamSynchronize(S(stream)));
+    if (pool_len > h->pool_cap) {
+        if (h->d_pool) {
+            (void)hipFree(h->d_pool);
+            h->bytes -= (int64_t)sizeof(double) * h->pool_cap;
+        }
+        h->d_pool = nullptr;
+        h->pool_cap = 0;
+        if (int e = dmalloc(h, &h->d_pool, sizeof(double) * pool_len)) return e;
+        h->pool_cap = pool_len;
+        for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+        h->graphs.clear();
+    }
+    if (pool_len)
+        LBM_HIP_TRY(hipMemcpy(h->d_pool, pool_host, sizeof(double) * pool_len,
+                              hipMemcpyHostToDevice));
+    // rows row0 .. row0+nrows-1 of the ring: at most two contiguous pieces
+    int r = 0;
+    while (r < nrows) {
+        const int row = row_of(h, (int64_t)row0 + r);
+        const int len = std::min(nrows - r, h->ring - row);
+        LBM_HIP_TRY(hipMemcpy(h->d_inlet_off + 5 * row, off_host + 5 * r, 5 * sizeof(int) * len,
+                              hipMemcpyHostToDevice));
+        r += len;
+    }
+    return LBM_OK;
+}
+
+extern "C" int lbm_init(lbm_fused *h, const double *g, double *rho, double *u, void *stream)
+{
+    if (!h || !g || ((rho == nullptr) != (u == nullptr)))
+        return set_error(LBM_ERR_ARG, "lbm_init: bad argument");
+    Params p = base_params(h);
+    p.g0 = g;
+    p.dst = h->buf[0];
+    p.rho_out = rho;
+    p.u_out = u;
+    launch_fused<MODE_INIT>(h, p, S(stream));
+    LBM_LAUNCH_CHECK();
+    return LBM_OK;
+}
+
+extern "C" int lbm_step(lbm_fused *h, int64_t it, double *rho, double *u, void *stream)
+{
+    if (!h || it < 1 || ((rho == nullptr) != (u == nullptr)))
+        return set_error(LBM_ERR_ARG, "lbm_step: bad argument");
+    if (!h->d_pool) return set_error(LBM_ERR_ARG, "lbm_step: no inlet table");
+    Params p = base_params(h);
+    p.src = h->buf[(it - 1) & 1];
+    p.dst = h->buf[it & 1];
+    p.k = row_of(h, it - 1);
+    p.rho_out = rho;
+    p.u_out = u;
+    launch_fused<MODE_STEP>(h, p, S(stream));
+    LBM_LAUNCH_CHECK();
+    return LBM_OK;
+}
+
+extern "C" int lbm_steps(lbm_fused *h, int64_t it0, int n, void *stream)
+{
+    if (!h || it0 < 1 || n < 0) return set_error(LBM_ERR_ARG, "lbm_steps: bad argument");
+    if (n == 0) return LBM_OK;
+    if (!h->d_pool) return set_error(LBM_ERR_ARG, "lbm_steps: no inlet table");
+    const GraphKey key{n, (int)(it0 & 1)};
+    auto itg = h->graphs.find(key);
+    if (itg == h->graphs.end()) {
+        hipGraph_t graph;
+        LBM_HIP_TRY(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < n; ++k) {
+            Params p = base_params(h);
+            p.src = h->buf[(it0 + k - 1) & 1];
+            p.dst = h->buf[(it0 + k) & 1];
+            p.base = h->d_base;
+            p.k = k;
+            launch_fused<MODE_STEP>(h, p, h->cap_stream);
+        }
+        hipError_t ec = hipStreamEndCapture(h->cap_stream, &graph);
+        if (ec != hipSuccess)
+            return set_error(LBM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+        hipGraphExec_t exec;
+        ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ec != hipSuccess)
+            return set_error(LBM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ec));
+        itg = h->graphs.emplace(key, exec).first;
+    }
+    k_set_int<<<1, 64, 0, S(stream)>>>(h->d_base, row_of(h, it0 - 1));
+    LBM_LAUNCH_CHECK();
+    LBM_HIP_TRY(hipGraphLaunch(itg->second, S(stream)));
+    return LBM_OK;
+}
+
+extern "C" int lbm_epilogue(lbm_fused *h, int64_t it, double *g, double *gup, double *rho,
+                            double *u, void *stream)
+{
+    if (!h || it < 0 || !g || !gup || ((rho == nullptr) != (u == nullptr)))
+        return set_error(LBM_ERR_ARG, "lbm_epilogue: bad argument");
+    if (!h->d_pool) return set_error(LBM_ERR_ARG, "lbm_epilogue: no inlet table");
+    Params p = base_params(h);
+    p.src = h->buf[it & 1];
+    p.g_out = g;
+    p.gup_out = gup;
+    p.k = row_of(h, it);
+    p.rho_out = rho;
+    p.u_out = u;
+    launch_fused<MODE_EPILOGUE>(h, p, S(stream));
+    LBM_LAUNCH_CHECK();
+    return LBM_OK;
+}
+
+extern "C" int lbm_drag_only(lbm_fused *h, int64_t it, void *stream)
+{
+    if (!h || it < 0) return set_error(LBM_ERR_ARG, "lbm_drag_only: bad argument");
+    if (!h->d_pool) return set_error(LBM_ERR_ARG, "lbm_drag_only: no inlet table");
+    Params p = base_params(h);
+    p.src = h->buf[it & 1];
+    p.k = row_of(h, it);
+    launch_drag_only(h, p, S(stream));
+    LBM_LAUNCH_CHECK();
+    return LBM_OK;
+}
+
+extern "C" int lbm_drag_read(lbm_fused *h, int64_t it0, int n, double *out, void *stream)
+{
+    if (!h || n < 0 || n > h->ring || it0 < 0 || (n && !out))
+        return set_error(LBM_ERR_ARG, "lbm_drag_read: bad argument");
+    if (n == 0 || h->n_obs == 0) return LBM_OK;
+    const int need = n * h->n_obs * 2;
+    if (need > h->drag_out_cap) {
+        LBM_HIP_TRY(hipStreamSynchronize(S(stream)));
+        if (h->d_drag_out) {
+            (void)hipFree(h->d_drag_out);
+            h->bytes -= (int64_t)sizeof(double) * h->drag_out_cap;
+        }
+        h->d_drag_out = nullptr;
+        h->drag_out_cap = 0;
+        if (int e = dmalloc(h, &h->d_drag_out, sizeof(double) * need)) return e;
+        h->drag_out_cap = need;
+    }
+    const int threads = n * h->n_obs;
+    k_drag_reduce<<<(threads + 127) / 128, 128, 0, S(stream)>>>(
+        h->d_drag_ring, h->n_links, h->ring, it0, n, h->d_obs_off, h->n_obs, h->d_plan_q,
+        h->d_drag_out);
+    LBM_LAUNCH_CHECK();
+    LBM_HIP_TRY(hipMemcpyAsync(out, h->d_drag_out, sizeof(double) * need, hipMemcpyDeviceToHost,
+                               S(stream)));
+    LBM_HIP_TRY(hipStreamSynchronize(S(stream)));
+    return LBM_OK;
+}
+
+extern "C" int lbm_fused_save(lbm_fused *h, int64_t it, void *stream)
+{
+    if (!h || it < 0) return set_error(LBM_ERR_ARG, "lbm_fused_save: bad argument");
+    const size_t sb = store_bytes(h->storage) * 9 * (size_t)h->N;
+    if (!h->ckpt)
+        if (int e = dmalloc(h, &h->ckpt, sb)) return e;
+    LBM_HIP_TRY(hipMemcpyAsync(h->ckpt, h->buf[it & 1], sb, hipMemcpyDeviceToDevice, S(stream)));
+    h->ckpt_it = it;
+    return LBM_OK;
+}
+
+extern "C" int lbm_fused_restore(lbm_fused *h, int64_t it, void *stream)
+{
+    if (!h || it < 0 || it != h->ckpt_it || !h->ckpt)
+        return set_error(LBM_ERR_ARG, "lbm_fused_restore: no checkpoint of that iteration");
+    const size_t sb = store_bytes(h->storage) * 9 * (size_t)h->N;
+    LBM_HIP_TRY(hipMemcpyAsync(h->buf[it & 1], h->ckpt, sb, hipMemcpyDeviceToDevice, S(stream)));
+    return LBM_OK;
+}
+
+extern "C" int lbm_mass(lbm_fused *h, int64_t it, double *out, void *stream)
+{
+    if (!h || it < 0 || !out) return set_error(LBM_ERR_ARG, "lbm_mass: bad argument");
+    const int nb = 1024;
+    if (!h->d_part) {
+        if (int e = dmalloc(h, &h->d_part, sizeof(double) * (nb + 1))) return e;
+        h->n_part = nb;
+    }
+    const void *b = h->buf[it & 1];
+    switch (h->storage) {
+    case LBM_STORE_F64: k_mass_partial<LBM_STORE_F64><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
+    case LBM_STORE_F32: k_mass_partial<LBM_STORE_F32><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
+    case LBM_STORE_F16: k_mass_partial<LBM_STORE_F16><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
+    default: k_mass_partial<LBM_STORE_BF16><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
+    }
+    LBM_LAUNCH_CHECK();
+    k_sum_serial<<<1, 64, 0, S(stream)>>>(h->d_part, nb, h->d_part + nb);
+    LBM_LAUNCH_CHECK();
+    LBM_HIP_TRY(hipMemcpyAsync(out, h->d_part + nb, sizeof(double), hipMemcpyDeviceToHost,
+                               S(stream)));
+    LBM_HIP_TRY(hipStreamSynchronize(S(stream)));
+    return LBM_OK;
+}

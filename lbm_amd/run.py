@@ -1,0 +1,291 @@
+"""The time loop (lbm/src/core/run.py:12-61) over the fused HIP time step.
+
+``run(lattice, app)`` keeps the reference's contract: the app's hooks are
+called in the reference order for every iteration, and on return the lattice
+holds exactly what the reference leaves (g, g_up and g_eq of the last
+iteration, rho/u = moments of the last iteration with the Zou-He values on
+the perimeter).  Underneath, iterations run in chunks on the GPU:
+
+1. ``app.set_bc`` is recorded once into a boundary-condition plan (obstacle
+   links, walls, corners); the per-call kernels are not launched.
+2. For a chunk of iterations [it0, it1): ``app.set_inlets`` is called for
+   each of them and the wall arrays are captured into an inlet table; the
+   iterations then run as fused launches (one per iteration, replayed from a
+   HIP graph), with the drag/lift terms of every iteration kept on the device.
+3. The hooks of those iterations are then replayed on the host in order:
+   printings, outputs (which see rho/u of it0, where chunks start), observables
+   (``lattice.drag_lift`` returns the value of that iteration), check_stop.
+4. If check_stop ends the run inside the chunk, the state is restored from
+   the chunk-start checkpoint and re-advanced to the stopping iteration.
+Chunks start at multiples of ``app.output_freq``; an app without it runs one
+iteration per chunk.  A boundary plan the fused kernel cannot express runs the
+reference's call-by-call loop on the per-call kernels instead.
+"""
+import time
+
+import numpy as np
+
+from lbm_amd import _lib
+from lbm_amd.domain import FusedDomain, pack_inlets
+
+_SIDES = ("u_bot", "u_left", "u_top", "u_right", "rho_right")
+_WALLS = {"bottom": 0, "left": 1, "top": 2, "right": 3, "right_pressure": 3}
+_CORNERS = ("bottom_left", "top_left", "top_right", "bottom_right")
+
+
+class PlanError(ValueError):
+    """The recorded set_bc plan has no fused equivalent."""
+
+
+def record_plan(lattice, app):
+    lattice._recording = []
+    try:
+        app.set_bc(lattice)
+        return lattice._recording
+    finally:
+        lattice._recording = None
+
+
+def compile_plan(plan):
+    """Map a recorded set_bc sequence onto the fused kernel's plan."""
+    bc = [0, 0, 0, 0]
+    corners = set()
+    obstacles = []
+    seen_wall = False
+    obs_before = obs_after = False
+    for kind, arg in plan:
+        if kind == "obstacle":
+            if any(o is arg for o in obstacles):
+                raise PlanError("an obstacle appears twice in set_bc")
+            obstacles.append(arg)
+            if corners:
+                raise PlanError("obstacle after the corners")
+            if seen_wall:
+                obs_after = True
+            else:
+                obs_before = True
+        elif kind in _WALLS:
+            if corners:
+                raise PlanError("wall after a corner")
+            side = _WALLS[kind]
+            if bc[side]:
+                raise PlanError("a wall appears twice in set_bc")
+            bc[side] = _lib.BC_PRESSURE if kind == "right_pressure" else _lib.BC_VELOCITY
+            seen_wall = True
+        elif kind in _CORNERS:
+            if kind in corners:
+                raise PlanError("a corner appears twice in set_bc")
+            corners.add(kind)
+        else:
+            raise PlanError("unknown boundary call " + kind)
+    if obs_before and obs_after:
+        raise PlanError("obstacles both before and after the walls")
+    if not all(bc) or len(corners) != 4:
+        raise PlanError("the fused plan needs all four walls and all four corners")
+    return bc, obstacles, (not obs_after)
+
+
+class _InletLog:
+    """Deduplicated snapshots of the five wall arrays, one row per iteration."""
+
+    def __init__(self, lattice):
+        self.lat = lattice
+        self.last = None      # tuple of arrays of the previous iteration
+
+    def snapshot(self):
+        cur = []
+        for i, name in enumerate(_SIDES):
+            a = getattr(self.lat, name)
+            prev = None if self.last is None else self.last[i]
+            if prev is not None and prev.shape == a.shape and np.array_equal(prev, a):
+                cur.append(prev)
+            else:
+                cur.append(np.array(a, dtype=np.float64, copy=True))
+        self.last = tuple(cur)
+        return self.last
+
+
+class FusedRun:
+    """Drives ``app`` on ``lattice`` with the fused time step."""
+
+    def __init__(self, lattice, app, chunk=1000, cells_per_thread=0):
+        import torch
+        self.lat = lattice
+        self.app = app
+        self.chunk = int(chunk)
+        lattice._need_gpu()
+        bc, obstacles, obs_first = compile_plan(record_plan(lattice, app))
+        self.obstacles = obstacles
+        self.dom = FusedDomain(lattice.nx, lattice.ny, lattice.om_p_lbm, lattice.om_m_lbm, bc,
+                               [(o.boundary, getattr(o, "ibb", None)) for o in obstacles],
+                               ibb=bool(lattice.IBB), obstacles_first=obs_first,
+                               storage=lattice.precision, ring=self.chunk + 2,
+                               cells_per_thread=cells_per_thread)
+        self.log = _InletLog(lattice)
+        self.prev_row = None
+        self.drag = None
+        self.drag_it0 = 0
+        n = (lattice.nx, lattice.ny)
+        self.rho_b = torch.empty(n, dtype=torch.float64, device="cuda")
+        self.u_b = torch.empty((2,) + n, dtype=torch.float64, device="cuda")
+
+    def close(self):
+        self.dom.close()
+
+    def _upload(self, it0, rows):
+        """Inlet rows for iterations it0-1 (repeated from the last chunk) .. it0+len-1."""
+        all_rows = ([self.prev_row] if it0 > 0 else []) + rows
+        pool, offs = pack_inlets(all_rows)
+        self.dom.set_inlets(it0 - 1 if it0 > 0 else 0, offs, pool)
+        self.prev_row = rows[-1]
+
+    def _advance(self, it0, it1, macro_last):
+        """Run iterations it0..it1-1: macro of it0 into lattice rho/u, of it1-1
+        into the chunk-end buffers (or lattice rho/u when macro_last)."""
+        dom = self.dom
+        f = self.lat._fields
+        rho, u = f["rho"].device(load=False), f["u"].device(load=False)
+        if it0 == 0:
+            dom.init(f["g"].device(), rho, u)
+        else:
+            dom.step(it0, rho, u)
+        f["rho"].written()
+        f["u"].written()
+        if it1 - it0 >= 3:
+            dom.steps(it0 + 1, it1 - it0 - 2)
+        if it1 - it0 >= 2:
+            rb, ub = (rho, u) if macro_last else (self.rho_b, self.u_b)
+            dom.step(it1 - 1, rb, ub)
+
+    def _finish(self, it_last):
+        """Epilogue of the last iteration: g, g_up, g_eq, rho/u as the reference leaves them."""
+        f = self.lat._fields
+        rho, u = f["rho"].device(), f["u"].device()
+        geq = f["g_eq"].device(load=False)
+        _lib.check(self.dom.L.lbm_equilibrium(self.lat.nx, self.lat.ny, u.data_ptr(),
+                                              rho.data_ptr(), geq.data_ptr(), _lib.stream_ptr()),
+                   "lbm_equilibrium")
+        self.dom.epilogue(it_last, f["g"].device(load=False), f["g_up"].device(load=False),
+                          rho, u)
+        for k in ("g_eq", "g", "g_up", "rho", "u"):
+            f[k].written()
+
+    def drag_sums(self, obs):
+        it = self.lat._replay.it
+        for k, o in enumerate(self.obstacles):
+            if o is obs:
+                row = self.drag[it - self.drag_it0, k]
+                return float(row[0]), float(row[1])
+        raise KeyError("drag_lift on an obstacle that set_bc does not bounce back")
+
+    # -- loop
+    def _chunk_end(self, it0):
+        app = self.app
+        it1 = it0 + self.chunk
+        freq = getattr(app, "output_freq", None)
+        if isinstance(freq, (int, np.integer)) and freq > 0:
+            it1 = min(it1, (it0 // freq + 1) * freq)
+        else:
+            it1 = it0 + 1
+        if getattr(app, "stop", None) == "it" and hasattr(app, "it_max"):
+            it1 = min(it1, int(app.it_max) + 1)
+        return max(it1, it0 + 1)
+
+    def go(self):
+        app, lat, dom = self.app, self.lat, self.dom
+        it0 = 0
+        while True:
+            it1 = self._chunk_end(it0)
+            rows = []
+            for it in range(it0, it1):
+                app.set_inlets(lat, it)
+                rows.append(self.log.snapshot())
+            self._upload(it0, rows)
+            if it0 > 0:
+                dom.save(it0 - 1)
+            self._advance(it0, it1, macro_last=False)
+            if self.obstacles:
+                dom.drag_only(it1 - 1)
+                self.drag, self.drag_it0 = dom.drag_read(it0, it1 - it0), it0
+            stop_it = self._replay(it0, it1)
+            if stop_it is None:
+                it0 = it1
+                continue
+            if stop_it < it1 - 1:
+                if it0 > 0:
+                    dom.restore(it0 - 1)
+                self._advance(it0, stop_it + 1, macro_last=True)
+            elif it1 - it0 >= 2:
+                f = lat._fields
+                f["rho"].device(load=False).copy_(self.rho_b)
+                f["u"].device(load=False).copy_(self.u_b)
+            self._finish(stop_it)
+            return stop_it
+
+    def _replay(self, it0, it1):
+        app, lat = self.app, self.lat
+        state = _Replay(self, it0)
+        lat._replay = state
+        try:
+            for it in range(it0, it1):
+                state.it = it
+                app.printings(it)
+                app.outputs(lat, it)
+                app.observables(lat, it)
+                if not app.check_stop(it):
+                    return it
+            return None
+        finally:
+            lat._replay = None
+
+
+class _Replay:
+    def __init__(self, run, it0):
+        self.run = run
+        self.it0 = it0
+        self.it = it0
+
+    def drag_sums(self, obs):
+        return self.run.drag_sums(obs)
+
+
+def _run_per_call(lattice, app):
+    """The reference loop on the per-call kernels (run.py:24-54)."""
+    it = 0
+    compute = True
+    while compute:
+        app.printings(it)
+        app.set_inlets(lattice, it)
+        lattice.macro()
+        app.outputs(lattice, it)
+        lattice.equilibrium()
+        lattice.collision_stream()
+        app.set_bc(lattice)
+        app.observables(lattice, it)
+        compute = app.check_stop(it)
+        it += 1
+    return it - 1
+
+
+def run(lattice, app, chunk=1000, fused=True):
+    """Run ``app`` on ``lattice`` (run.py:12-61); returns the last iteration."""
+    app.initialize(lattice)
+    start = time.time()
+    print('### Solving')
+    engine = None
+    if fused:
+        try:
+            engine = FusedRun(lattice, app, chunk=chunk)
+        except (PlanError, _lib.Unsupported) as e:
+            print('# fused time step unavailable (' + str(e) + '), using per-call kernels')
+            engine = None
+    if engine is not None:
+        try:
+            last = engine.go()
+        finally:
+            engine.close()
+    else:
+        last = _run_per_call(lattice, app)
+    print("# Loop time = {:f}".format(time.time() - start))
+    app.finalize(lattice)
+    return last
