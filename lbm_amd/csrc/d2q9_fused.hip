@@ -39,6 +39,9 @@ int set_error(int code, const std::string &msg)
 }
 
 // ---------------------------------------------------------------- storage --
+// ld/st move full populations f_q (the reference's variables).  The float
+// forms store the deviation h_q = f_q - w_q (IEEE half, bfloat16 or float);
+// ldh/sth move that deviation without conversion to f.
 template <int S> struct Store;
 
 template <> struct Store<LBM_STORE_F64> {
@@ -51,13 +54,23 @@ template <> struct Store<LBM_STORE_F64> {
 template <> struct Store<LBM_STORE_F32> {
     using s_t = float;
     using T = float;
-    static __device__ __forceinline__ T ld(const s_t *p, size_t i, int) { return p[i]; }
-    static __device__ __forceinline__ void st(s_t *p, size_t i, int, T v) { p[i] = v; }
+    static __device__ __forceinline__ float ldh(const s_t *p, size_t i) { return p[i]; }
+    static __device__ __forceinline__ void sth(s_t *p, size_t i, float h) { p[i] = h; }
+    static __device__ __forceinline__ T ld(const s_t *p, size_t i, int q)
+    {
+        return weight<float>(q) + p[i];
+    }
+    static __device__ __forceinline__ void st(s_t *p, size_t i, int q, T v)
+    {
+        p[i] = v - weight<float>(q);
+    }
 };
 
 template <> struct Store<LBM_STORE_F16> {
     using s_t = _Float16;
     using T = float;
+    static __device__ __forceinline__ float ldh(const s_t *p, size_t i) { return (float)p[i]; }
+    static __device__ __forceinline__ void sth(s_t *p, size_t i, float h) { p[i] = (_Float16)h; }
     static __device__ __forceinline__ T ld(const s_t *p, size_t i, int q)
     {
         return weight<float>(q) + (float)p[i];
@@ -68,18 +81,28 @@ template <> struct Store<LBM_STORE_F16> {
     }
 };
 
+__device__ __forceinline__ uint16_t f32_to_bf16(float v)
+{
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
 template <> struct Store<LBM_STORE_BF16> {
     using s_t = uint16_t;
     using T = float;
+    static __device__ __forceinline__ float ldh(const s_t *p, size_t i)
+    {
+        return __uint_as_float((uint32_t)p[i] << 16);
+    }
+    static __device__ __forceinline__ void sth(s_t *p, size_t i, float h) { p[i] = f32_to_bf16(h); }
     static __device__ __forceinline__ T ld(const s_t *p, size_t i, int q)
     {
-        return weight<float>(q) + __uint_as_float((uint32_t)p[i] << 16);
+        return weight<float>(q) + ldh(p, i);
     }
     static __device__ __forceinline__ void st(s_t *p, size_t i, int q, T v)
     {
-        uint32_t u = __float_as_uint(v - weight<float>(q));
-        u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-        p[i] = (uint16_t)(u >> 16);
+        p[i] = f32_to_bf16(v - weight<float>(q));
     }
 };
 
@@ -105,7 +128,7 @@ struct Params {
     double *rho_out, *u_out;
     int nx, ny, N;
     uint32_t magic;        // floor(2^32 / ny) + 1, for node -> (x, y)
-    double om_p, b0, a1, a2, a3, a4;
+    double om_p, om_m, b0, a1, a2, a3, a4;
     int bc_bottom, bc_left, bc_top, bc_right, corners;
     const double *pool;    // inlet side arrays
     const int *inlet_off;  // [ring][5] offsets into pool
@@ -114,6 +137,9 @@ struct Params {
     int k;                 // row = (*base + k) mod ring  -- row of iteration it-1
     const int *tile_slot;  // [n_tiles] -1 or slot of the tile's link offsets
     const int *cell_off;   // [n_slots][tile_cells+1]
+    const uint32_t *cell_mask; // [n_slots][tile_cells/32] bit set = node has links
+    int n_main;            // work-groups of interior tiles (the rest do the perimeter)
+    int n_perim;           // perimeter nodes
     const LinkRec *links;  // sorted by node
     double *drag_ring;     // [ring][n_links]
     int n_links;
@@ -339,13 +365,11 @@ __device__ void gather_bc(const Params &p, int x, int y, int cell, int row,
 }
 
 template <int S>
-__device__ __forceinline__ void store_drag(const Params &p, int cell, int row,
-                                           const typename Store<S>::T *f)
+__device__ __forceinline__ void store_drag_range(const Params &p, int cell, int row, int lo,
+                                                 int hi, const typename Store<S>::T *f)
 {
     using St = Store<S>;
     const auto *src = static_cast<const typename St::s_t *>(p.src);
-    int lo, hi;
-    link_range(p, cell, lo, hi);
     for (int k = lo; k < hi; ++k) {
         const LinkRec L = p.links[k];
         double fb = 0.0;
@@ -355,6 +379,15 @@ __device__ __forceinline__ void store_drag(const Params &p, int cell, int row,
         const double A = (double)St::ld(src, (size_t)L.q * p.N + cell, L.q);
         p.drag_ring[(size_t)row * p.n_links + L.dslot] = A + fb;
     }
+}
+
+template <int S>
+__device__ __forceinline__ void store_drag(const Params &p, int cell, int row,
+                                           const typename Store<S>::T *f)
+{
+    int lo, hi;
+    link_range(p, cell, lo, hi);
+    store_drag_range<S>(p, cell, row, lo, hi, f);
 }
 
 // Moments, equilibrium, TRT collision and store (lattice.py:178-189,
@@ -393,6 +426,54 @@ __device__ __forceinline__ void collide_store(const Params &p, int cell,
     }
 }
 
+// Float forms: moments, equilibrium and TRT collision on the deviations
+// h_q = f_q - w_q, with the collision written per opposite pair (q, qb):
+//   h+- = (h_q +- h_qb)/2,  E+ = w (drho + rho (4.5 cu^2 - 1.5 u^2)),
+//   E- = 3 w rho cu,  h_q' = h_q - om_p (h+ - E+) - om_m (h- - E-),
+//   h_qb' = h_qb - om_p (h+ - E+) + om_m (h- - E-),
+// which is nb.py:24-35 with g_eq of nb.py:7-17, rearranged (E = g_eq - w).
+template <int S>
+__device__ __forceinline__ void collide_store_h(const Params &p, int cell, const float *h,
+                                                bool macro)
+{
+#pragma clang fp contract(fast)
+    using St = Store<S>;
+    auto *dst = static_cast<typename St::s_t *>(p.dst);
+    const float drho = ((h[0] + h[1]) + (h[2] + h[3])) + ((h[4] + h[5]) + (h[6] + h[7])) + h[8];
+    const float jx = (h[1] - h[2]) + (h[5] - h[6]) + (h[8] - h[7]);
+    const float jy = (h[3] - h[4]) + (h[5] - h[6]) + (h[7] - h[8]);
+    const float rho = 1.0f + drho;
+    const float inv = __builtin_amdgcn_rcpf(rho);
+    const float ux = jx * inv, uy = jy * inv;
+    if (macro) {
+        p.rho_out[cell] = (double)rho;
+        p.u_out[cell] = (double)ux;
+        p.u_out[(size_t)p.N + cell] = (double)uy;
+    }
+    const float op = (float)p.om_p;
+    const float hp = 0.5f * op, hm = 0.5f * (float)p.om_m;
+    const float usq = 1.5f * (ux * ux + uy * uy);
+    const float base = drho - rho * usq;   // E+/w without the 4.5 cu^2 term
+    // rest population
+    const float e0 = weight<float>(0) * base;
+    St::sth(dst, cell, h[0] - op * (h[0] - e0));
+    const float wr3[2] = {3.0f * weight<float>(1) * rho, 3.0f * weight<float>(5) * rho};
+    const float w45[2] = {4.5f * weight<float>(1) * rho, 4.5f * weight<float>(5) * rho};
+    const float wb[2] = {weight<float>(1) * base, weight<float>(5) * base};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int q = 2 * k + 1, qb = q + 1;   // pairs (1,2) (3,4) (5,6) (7,8)
+        const int d = k < 2 ? 0 : 1;
+        const float cu = k == 0 ? ux : (k == 1 ? uy : (k == 2 ? ux + uy : uy - ux));
+        const float ep = wb[d] + w45[d] * (cu * cu);
+        const float em = wr3[d] * cu;
+        const float dp = hp * ((h[q] + h[qb]) - 2.0f * ep);
+        const float dm = hm * ((h[q] - h[qb]) - 2.0f * em);
+        St::sth(dst, (size_t)q * p.N + cell, h[q] - dp - dm);
+        St::sth(dst, (size_t)qb * p.N + cell, h[qb] - dp + dm);
+    }
+}
+
 // Work-group index -> tile index, so that each XCD (work-groups are dealt
 // round-robin over the 8 XCDs) sweeps one contiguous slab of the lattice and
 // the rows its tiles share stay in that XCD's L2.
@@ -401,6 +482,20 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
     const int q = nb >> 3, rr = nb & 7;
     const int xcd = b & 7, i = b >> 3;
     return xcd * q + min(xcd, rr) + i;
+}
+
+template <int S>
+__device__ __forceinline__ void collide_any(const Params &p, int cell,
+                                            typename Store<S>::T *f, bool macro)
+{
+    if constexpr (S == LBM_STORE_F64) {
+        collide_store<S>(p, cell, f, macro);
+    } else {
+        float h[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) h[q] = f[q] - weight<float>(q);
+        collide_store_h<S>(p, cell, h, macro);
+    }
 }
 
 template <int S, int MODE, int CPT>
@@ -414,6 +509,7 @@ __global__ __launch_bounds__(256) void k_fused(Params p)
     const int lx = p.nx - 1, ly = p.ny - 1;
     const int row = (MODE == MODE_INIT) ? 0 : bc_row(p);
     const bool macro = p.rho_out != nullptr;
+    const auto *src = static_cast<const typename St::s_t *>(p.src);
 #pragma unroll 1
     for (int c = 0; c < CPT; ++c) {
         const int cell = base + c * 256;
@@ -422,27 +518,53 @@ __global__ __launch_bounds__(256) void k_fused(Params p)
         if (MODE == MODE_INIT) {
 #pragma unroll
             for (int q = 0; q < 9; ++q) f[q] = T(p.g0[(size_t)q * p.N + cell]);
-            collide_store<S>(p, cell, f, macro);
+            collide_any<S>(p, cell, f, macro);
             continue;
         }
         const int x = node_x(p, cell), y = cell - x * p.ny;
         const bool interior = (unsigned)(x - 1) < (unsigned)(lx - 1) &&
                               (unsigned)(y - 1) < (unsigned)(ly - 1);
-        if (MODE == MODE_STEP && interior && !link_tile) {
-            const auto *src = static_cast<const typename St::s_t *>(p.src);
+        if (MODE == MODE_STEP && interior) {
+            // interior node: unchecked pull; plane q is read at the constant
+            // node offset -(cx*ny + cy) from a work-group-uniform base
+            int lo = 0, hi = 0;
+            if (link_tile) link_range(p, cell, lo, hi);
+            if constexpr (S == LBM_STORE_F64) {
 #pragma unroll
-            for (int q = 0; q < 9; ++q)
-                f[q] = St::ld(src, q * p.N + cell - (CX[q] * p.ny + CY[q]), q);
-            collide_store<S>(p, cell, f, macro);
+                for (int q = 0; q < 9; ++q) {
+                    const auto *pq = src + ((ptrdiff_t)q * p.N - (CX[q] * p.ny + CY[q]));
+                    f[q] = pq[cell];
+                }
+                if (lo < hi) {
+                    apply_links<S>(p, cell, lo, hi, f);
+                    store_drag_range<S>(p, cell, row, lo, hi, f);
+                }
+                collide_store<S>(p, cell, f, macro);
+            } else {
+                float h[9];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    const auto *pq = src + ((ptrdiff_t)q * p.N - (CX[q] * p.ny + CY[q]));
+                    h[q] = St::ldh(pq, cell);
+                }
+                if (lo < hi) {
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) f[q] = h[q] + weight<float>(q);
+                    apply_links<S>(p, cell, lo, hi, f);
+                    store_drag_range<S>(p, cell, row, lo, hi, f);
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) h[q] = f[q] - weight<float>(q);
+                }
+                collide_store_h<S>(p, cell, h, macro);
+            }
             continue;
         }
         T r = T(0), ux = T(0), uy = T(0);
         gather_bc<S>(p, x, y, cell, row, f, r, ux, uy);
         if (link_tile) store_drag<S>(p, cell, row, f);
         if (MODE == MODE_STEP) {
-            collide_store<S>(p, cell, f, macro);
+            collide_any<S>(p, cell, f, macro);
         } else {  // MODE_EPILOGUE
-            const auto *src = static_cast<const typename St::s_t *>(p.src);
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
                 p.g_out[(size_t)q * p.N + cell] = (double)f[q];
@@ -454,6 +576,97 @@ __global__ __launch_bounds__(256) void k_fused(Params p)
                 p.u_out[(size_t)p.N + cell] = (double)uy;
             }
         }
+    }
+}
+
+// Perimeter node i (bottom row, top row, then the left and right columns
+// without their corners) -> (x, y).
+__device__ __forceinline__ void perim_node(const Params &p, int i, int &x, int &y)
+{
+    const int nx = p.nx, ny = p.ny;
+    if (i < nx) { x = i; y = 0; return; }
+    i -= nx;
+    if (i < nx) { x = i; y = ny - 1; return; }
+    i -= nx;
+    if (i < ny - 2) { x = 0; y = i + 1; return; }
+    i -= ny - 2;
+    x = nx - 1;
+    y = i + 1;
+}
+
+// The time step.  Work-groups [0, n_main) sweep interior tiles of 256*CPT
+// nodes: every work-item issues the 9*CPT pull loads of its nodes first
+// (perimeter nodes of the tile load a valid interior node and are skipped),
+// then fixes up obstacle-link nodes (tiles with links only) and collides.
+// Work-groups [n_main, grid) apply the Zou-He perimeter plan node by node.
+template <int S, int CPT>
+__global__ __launch_bounds__(256) void k_step(Params p)
+{
+    using St = Store<S>;
+    using T = typename St::T;
+    using s_t = typename St::s_t;
+    const int row = bc_row(p);
+    const bool macro = p.rho_out != nullptr;
+    const s_t *__restrict__ src = static_cast<const s_t *>(p.src);
+    if ((int)blockIdx.x >= p.n_main) {
+        const int i = ((int)blockIdx.x - p.n_main) * 256 + threadIdx.x;
+        if (i >= p.n_perim) return;
+        int x, y;
+        perim_node(p, i, x, y);
+        const int cell = x * p.ny + y;
+        T f[9], r = T(0), ux = T(0), uy = T(0);
+        gather_bc<S>(p, x, y, cell, row, f, r, ux, uy);
+        store_drag<S>(p, cell, row, f);
+        collide_any<S>(p, cell, f, macro);
+        return;
+    }
+    const int tile = xcd_tile(blockIdx.x, p.n_main);
+    const int slot = p.tile_slot ? p.tile_slot[tile] : -1;
+    const int base = tile * (256 * CPT) + threadIdx.x;
+    const int lx = p.nx - 1, ly = p.ny - 1;
+    T v[CPT][9];
+    bool in[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const int cell = base + c * 256;
+        const int x = node_x(p, cell), y = cell - x * p.ny;
+        in[c] = cell < p.N && (unsigned)(x - 1) < (unsigned)(lx - 1) &&
+                (unsigned)(y - 1) < (unsigned)(ly - 1);
+        const int lc = in[c] ? cell : p.ny + 1;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const s_t *pq = src + ((ptrdiff_t)q * p.N - (CX[q] * p.ny + CY[q]));
+            if constexpr (S == LBM_STORE_F64)
+                v[c][q] = pq[lc];
+            else
+                v[c][q] = St::ldh(pq, lc);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        if (!in[c]) continue;
+        const int cell = base + c * 256;
+        if (slot >= 0) {
+            const int local = cell & ((256 * CPT) - 1);
+            const uint32_t word = p.cell_mask[(size_t)slot * (8 * CPT) + (local >> 5)];
+            if ((word >> (local & 31)) & 1u) {
+                int lo, hi;
+                link_range(p, cell, lo, hi);
+                T f[9];
+#pragma unroll
+                for (int q = 0; q < 9; ++q)
+                    f[q] = (S == LBM_STORE_F64) ? v[c][q] : v[c][q] + weight<float>(q);
+                apply_links<S>(p, cell, lo, hi, f);
+                store_drag_range<S>(p, cell, row, lo, hi, f);
+#pragma unroll
+                for (int q = 0; q < 9; ++q)
+                    v[c][q] = (S == LBM_STORE_F64) ? f[q] : f[q] - weight<float>(q);
+            }
+        }
+        if constexpr (S == LBM_STORE_F64)
+            collide_store<S>(p, cell, v[c], macro);
+        else
+            collide_store_h<S>(p, cell, v[c], macro);
     }
 }
 
@@ -542,7 +755,7 @@ using namespace lbm;
 struct lbm_fused {
     int nx = 0, ny = 0, N = 0, storage = 0, cpt = 1, tile_cells = 256, tile_shift = 8, n_tiles = 0;
     uint32_t magic = 0;
-    double om_p = 0, b0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+    double om_p = 0, om_m = 0, b0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
     int bc[4] = {0, 0, 0, 0};
     int corners = 0;
     int n_links = 0, n_obs = 0, ring = 0, n_slots = 0, n_link_cells = 0;
@@ -550,6 +763,8 @@ struct lbm_fused {
     void *ckpt = nullptr;
     int64_t ckpt_it = -1;
     int *d_tile_slot = nullptr, *d_cell_off = nullptr, *d_link_cells = nullptr;
+    uint32_t *d_cell_mask = nullptr;
+    int n_perim = 0;
     LinkRec *d_links = nullptr;
     int *d_obs_off = nullptr;
     int8_t *d_plan_q = nullptr;
@@ -586,6 +801,7 @@ Params base_params(const lbm_fused *h)
     p.N = h->N;
     p.magic = h->magic;
     p.om_p = h->om_p;
+    p.om_m = h->om_m;
     p.b0 = h->b0;
     p.a1 = h->a1;
     p.a2 = h->a2;
@@ -603,6 +819,9 @@ Params base_params(const lbm_fused *h)
     p.k = 0;
     p.tile_slot = h->n_slots ? h->d_tile_slot : nullptr;
     p.cell_off = h->d_cell_off;
+    p.cell_mask = h->d_cell_mask;
+    p.n_main = h->n_tiles;
+    p.n_perim = h->n_perim;
     p.links = h->d_links;
     p.drag_ring = h->d_drag_ring;
     p.n_links = h->n_links;
@@ -627,6 +846,27 @@ void launch_fused_s(const lbm_fused *h, const Params &p, hipStream_t s)
     case 2: k_fused<S, MODE, 2><<<grid, 256, 0, s>>>(p); break;
     case 4: k_fused<S, MODE, 4><<<grid, 256, 0, s>>>(p); break;
     default: k_fused<S, MODE, 8><<<grid, 256, 0, s>>>(p); break;
+    }
+}
+
+template <int S> void launch_step_s(const lbm_fused *h, const Params &p, hipStream_t s)
+{
+    const int grid = h->n_tiles + (h->n_perim + 255) / 256;
+    switch (h->cpt) {
+    case 1: k_step<S, 1><<<grid, 256, 0, s>>>(p); break;
+    case 2: k_step<S, 2><<<grid, 256, 0, s>>>(p); break;
+    case 4: k_step<S, 4><<<grid, 256, 0, s>>>(p); break;
+    default: k_step<S, 8><<<grid, 256, 0, s>>>(p); break;
+    }
+}
+
+void launch_step(const lbm_fused *h, const Params &p, hipStream_t s)
+{
+    switch (h->storage) {
+    case LBM_STORE_F64: launch_step_s<LBM_STORE_F64>(h, p, s); break;
+    case LBM_STORE_F32: launch_step_s<LBM_STORE_F32>(h, p, s); break;
+    case LBM_STORE_F16: launch_step_s<LBM_STORE_F16>(h, p, s); break;
+    default: launch_step_s<LBM_STORE_BF16>(h, p, s); break;
     }
 }
 
@@ -747,12 +987,22 @@ int build_links(lbm_fused *h, const lbm_fused_desc *d)
     }
     h->n_slots = slot;
     h->n_link_cells = (int)link_cells.size();
+    // one bit per node of each link tile: the node has links
+    std::vector<uint32_t> mask((size_t)std::max(slot, 1) * (tc / 32), 0u);
+    for (const auto &L : recs) {
+        const int sl = tile_slot[L.cell >> h->tile_shift];
+        const int local = L.cell & (tc - 1);
+        mask[(size_t)sl * (tc / 32) + (local >> 5)] |= 1u << (local & 31);
+    }
     int e;
     if ((e = dmalloc(h, &h->d_tile_slot, sizeof(int) * h->n_tiles))) return e;
     if ((e = dmalloc(h, &h->d_cell_off, sizeof(int) * cell_off.size()))) return e;
     if ((e = dmalloc(h, &h->d_links, sizeof(LinkRec) * nl))) return e;
     if ((e = dmalloc(h, &h->d_link_cells, sizeof(int) * link_cells.size()))) return e;
     if ((e = dmalloc(h, &h->d_plan_q, nl))) return e;
+    if ((e = dmalloc(h, &h->d_cell_mask, sizeof(uint32_t) * mask.size()))) return e;
+    LBM_HIP_TRY(hipMemcpy(h->d_cell_mask, mask.data(), sizeof(uint32_t) * mask.size(),
+                          hipMemcpyHostToDevice));
     LBM_HIP_TRY(hipMemcpy(h->d_tile_slot, tile_slot.data(), sizeof(int) * h->n_tiles,
                           hipMemcpyHostToDevice));
     if (!cell_off.empty())
@@ -783,7 +1033,7 @@ void free_all(lbm_fused *h)
 {
     for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
     h->graphs.clear();
-    void *ptrs[] = {h->buf[0], h->buf[1], h->ckpt, h->d_tile_slot, h->d_cell_off,
+    void *ptrs[] = {h->buf[0], h->buf[1], h->ckpt, h->d_tile_slot, h->d_cell_off, h->d_cell_mask,
                     h->d_link_cells, h->d_links, h->d_obs_off, h->d_plan_q, h->d_drag_ring,
                     h->d_drag_out, h->d_pool, h->d_inlet_off, h->d_base, h->d_zero, h->d_part};
     for (void *p : ptrs)
@@ -826,14 +1076,16 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     h->N = d->nx * d->ny;
     h->storage = d->storage;
     h->cpt = d->cells_per_thread;
-    if (h->cpt == 0) h->cpt = 2;
+    if (h->cpt == 0) h->cpt = (d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16) ? 4 : 2;
     if (h->cpt != 1 && h->cpt != 2 && h->cpt != 4 && h->cpt != 8)
         return set_error(LBM_ERR_ARG, "cells_per_thread must be 1, 2, 4 or 8");
     h->tile_cells = 256 * h->cpt;
     h->tile_shift = 8 + (h->cpt == 1 ? 0 : (h->cpt == 2 ? 1 : (h->cpt == 4 ? 2 : 3)));
     h->n_tiles = (h->N + h->tile_cells - 1) / h->tile_cells;
+    h->n_perim = 2 * d->nx + 2 * d->ny - 4;
     h->magic = (uint32_t)((1ULL << 32) / (uint64_t)d->ny) + 1u;
     h->om_p = d->om_p;
+    h->om_m = d->om_m;
     // the scalar factors of nb.py:25 and nb.py:32-35
     h->b0 = 1.0 - d->om_p;
     h->a1 = 1.0 - 0.5 * (d->om_p + d->om_m);
@@ -948,7 +1200,7 @@ extern "C" int lbm_step(lbm_fused *h, int64_t it, double *rho, double *u, void *
     p.k = row_of(h, it - 1);
     p.rho_out = rho;
     p.u_out = u;
-    launch_fused<MODE_STEP>(h, p, S(stream));
+    launch_step(h, p, S(stream));
     LBM_LAUNCH_CHECK();
     return LBM_OK;
 }
@@ -969,7 +1221,7 @@ extern "C" int lbm_steps(lbm_fused *h, int64_t it0, int n, void *stream)
             p.dst = h->buf[(it0 + k) & 1];
             p.base = h->d_base;
             p.k = k;
-            launch_fused<MODE_STEP>(h, p, h->cap_stream);
+            launch_step(h, p, h->cap_stream);
         }
         hipError_t ec = hipStreamEndCapture(h->cap_stream, &graph);
         if (ec != hipSuccess)

@@ -93,8 +93,8 @@ def test_bounce_back_and_drag(ibb):
     solid[15:20, 10:15] = True
     C = np.array([[0, 0], [1, 0], [-1, 0], [0, 1], [0, -1], [1, 1], [-1, -1], [-1, 1], [1, -1]])
     bnd = []
-    for i in range(NX):
-        for j in range(NY):
+    for i in range(1, NX - 1):
+        for j in range(1, NY - 1):
             if solid[i, j]:
                 continue
             for q in range(1, 9):
