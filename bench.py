@@ -128,23 +128,25 @@ def run_form(app, lat, om_p, om_m, storage, steps, warmup, graph_len, cpt=0):
                 n_links=int(sum(len(o.boundary) for o in app.obstacles)))
 
 
-def copy_ceiling_gbs():
-    """Device-to-device copy of 2 GiB with torch (read + write bytes / time)."""
+def copy_ceiling_gbs(gib=2, blocks=4096, reps=10):
+    """HBM copy ceiling: lbm_copy_probe moving `gib` GiB (read + write bytes / time)."""
     import torch
-    n = 1 << 28
-    a = torch.empty(n, dtype=torch.float64, device="cuda").fill_(1.0)
+    from lbm_amd import _lib
+    L = _lib.load()
+    nbytes = gib << 30
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
+    st = _lib.stream_ptr()
     for _ in range(3):
-        b.copy_(a)
+        _lib.check(L.lbm_copy_probe(b.data_ptr(), a.data_ptr(), nbytes, blocks, st))
     torch.cuda.synchronize()
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s.record()
-    reps = 10
     for _ in range(reps):
-        b.copy_(a)
+        _lib.check(L.lbm_copy_probe(b.data_ptr(), a.data_ptr(), nbytes, blocks, st))
     e.record()
     torch.cuda.synchronize()
-    gbs = 2 * n * 8 * reps / (s.elapsed_time(e) * 1e-3) / 1e9
+    gbs = 2 * nbytes * reps / (s.elapsed_time(e) * 1e-3) / 1e9
     del a, b
     return gbs
 

@@ -38,7 +38,7 @@ extern "C" {
 
 /* Population storage of the fused path. */
 #define LBM_STORE_F64   0   /* double storage, double arithmetic */
-#define LBM_STORE_F32   1   /* float storage, float arithmetic */
+#define LBM_STORE_F32   1   /* float storage of f_q - w_q, float arithmetic */
 #define LBM_STORE_F16   2   /* IEEE half storage of f_q - w_q, float arithmetic */
 #define LBM_STORE_BF16  3   /* bfloat16 storage of f_q - w_q, float arithmetic */
 
@@ -153,6 +153,8 @@ typedef struct {
     const int32_t *obs_offsets_host; /* [n_obs+1] link range of each obstacle */
     int32_t ring;               /* rows of the inlet table and of the drag ring */
     int32_t cells_per_thread;   /* 1, 2, 4 or 8 (0 = library default) */
+    int32_t vector_width;       /* 16-bit forms: 4 = four consecutive nodes per
+                                   work-item (default when ny % 4 == 0), 1 = one */
 } lbm_fused_desc;
 
 int  lbm_fused_create(const lbm_fused_desc *desc, lbm_fused **out);
@@ -203,6 +205,15 @@ int lbm_mass(lbm_fused *h, int64_t it, double *out_host, void *stream);
 
 /* Bytes of device memory held by the handle. */
 int64_t lbm_fused_device_bytes(const lbm_fused *h);
+
+/* ------------------------------------------------------------------------
+ * Measurement utility (no counterpart in the reference): device-to-device
+ * copy of `bytes` (a multiple of 16) with `blocks` work-groups of 256
+ * work-items and 16-byte accesses -- the HBM ceiling the time step is
+ * compared with, and the kernel of known traffic that calibrates the
+ * FETCH_SIZE/WRITE_SIZE counters.
+ * ---------------------------------------------------------------------- */
+int lbm_copy_probe(void *dst_dev, const void *src_dev, int64_t bytes, int blocks, void *stream);
 
 #ifdef __cplusplus
 }

@@ -40,6 +40,7 @@ class FusedDesc(ctypes.Structure):
         ("obs_offsets_host", ctypes.POINTER(c_i32)),
         ("ring", c_i32),
         ("cells_per_thread", c_i32),
+        ("vector_width", c_i32),
     ]
 
 
@@ -74,6 +75,7 @@ _SIGNATURES = {
     "lbm_fused_restore": ([c_vp, c_i64, c_vp], c_int),
     "lbm_mass": ([c_vp, c_i64, c_vp, c_vp], c_int),
     "lbm_fused_device_bytes": ([c_vp], c_i64),
+    "lbm_copy_probe": ([c_vp, c_vp, c_i64, c_int, c_vp], c_int),
 }
 
 _lib = None

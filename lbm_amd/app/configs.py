@@ -78,7 +78,7 @@ def place_squares(nx, ny, n_obs, seed, side_lo=32, side_hi=96, margin=4, x_lo=No
 
 class obstacle_array(channel_app):
     def __init__(self, ny=2048, aspect=4, n_obs=64, seed=0, precision='f16', iterations=10000,
-                 sigma=1000, n_spts=400):
+                 sigma=1000, n_spts=400, side_lo=32, side_hi=96):
         self.name = 'obstacle_array'
         self.Re_lbm = 2000.0
         self.L_lbm = ny
@@ -108,7 +108,7 @@ class obstacle_array(channel_app):
         self.sigma = sigma
         gx = (self.x_max - self.x_min) / (self.nx - 1)
         gy = (self.y_max - self.y_min) / (self.ny - 1)
-        self.squares = place_squares(self.nx, self.ny, n_obs, seed)
+        self.squares = place_squares(self.nx, self.ny, n_obs, seed, side_lo, side_hi)
         self.obstacles = []
         for k, (cx, cy, side) in enumerate(self.squares):
             pos = [self.x_min + cx * gx, self.y_min + cy * gy]

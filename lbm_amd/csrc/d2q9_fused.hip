@@ -432,46 +432,68 @@ __device__ __forceinline__ void collide_store(const Params &p, int cell,
 //   E- = 3 w rho cu,  h_q' = h_q - om_p (h+ - E+) - om_m (h- - E-),
 //   h_qb' = h_qb - om_p (h+ - E+) + om_m (h- - E-),
 // which is nb.py:24-35 with g_eq of nb.py:7-17, rearranged (E = g_eq - w).
-template <int S>
-__device__ __forceinline__ void collide_store_h(const Params &p, int cell, const float *h,
-                                                bool macro)
+struct Coll {
+    float op, hp, hm;  // om_p, om_p/2, om_m/2
+};
+
+__device__ __forceinline__ Coll coll_consts(const Params &p)
+{
+    return Coll{(float)p.om_p, 0.5f * (float)p.om_p, 0.5f * (float)p.om_m};
+}
+
+// In place: h (deviations of one node) -> post-collision deviations.
+__device__ __forceinline__ void collide_h(const Coll &k, float *h, float &rho, float &ux,
+                                          float &uy)
 {
 #pragma clang fp contract(fast)
-    using St = Store<S>;
-    auto *dst = static_cast<typename St::s_t *>(p.dst);
     const float drho = ((h[0] + h[1]) + (h[2] + h[3])) + ((h[4] + h[5]) + (h[6] + h[7])) + h[8];
     const float jx = (h[1] - h[2]) + (h[5] - h[6]) + (h[8] - h[7]);
     const float jy = (h[3] - h[4]) + (h[5] - h[6]) + (h[7] - h[8]);
-    const float rho = 1.0f + drho;
+    rho = 1.0f + drho;
     const float inv = __builtin_amdgcn_rcpf(rho);
-    const float ux = jx * inv, uy = jy * inv;
+    ux = jx * inv;
+    uy = jy * inv;
+    const float usq = 1.5f * (ux * ux + uy * uy);
+    const float base = drho - rho * usq;   // E+/w without the 4.5 cu^2 term
+    const float e0 = weight<float>(0) * base;
+    h[0] = h[0] - k.op * (h[0] - e0);
+    const float wr3[2] = {3.0f * weight<float>(1) * rho, 3.0f * weight<float>(5) * rho};
+    const float w45[2] = {4.5f * weight<float>(1) * rho, 4.5f * weight<float>(5) * rho};
+    const float wb[2] = {weight<float>(1) * base, weight<float>(5) * base};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int q = 2 * j + 1, qb = q + 1;   // pairs (1,2) (3,4) (5,6) (7,8)
+        const int d = j < 2 ? 0 : 1;
+        const float cu = j == 0 ? ux : (j == 1 ? uy : (j == 2 ? ux + uy : uy - ux));
+        const float ep = wb[d] + w45[d] * (cu * cu);
+        const float em = wr3[d] * cu;
+        const float dp = k.hp * ((h[q] + h[qb]) - 2.0f * ep);
+        const float dm = k.hm * ((h[q] - h[qb]) - 2.0f * em);
+        const float a = h[q] - dp - dm;
+        const float bb = h[qb] - dp + dm;
+        h[q] = a;
+        h[qb] = bb;
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void collide_store_h(const Params &p, int cell, const float *hin,
+                                                bool macro)
+{
+    using St = Store<S>;
+    auto *dst = static_cast<typename St::s_t *>(p.dst);
+    float h[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) h[q] = hin[q];
+    float rho, ux, uy;
+    collide_h(coll_consts(p), h, rho, ux, uy);
     if (macro) {
         p.rho_out[cell] = (double)rho;
         p.u_out[cell] = (double)ux;
         p.u_out[(size_t)p.N + cell] = (double)uy;
     }
-    const float op = (float)p.om_p;
-    const float hp = 0.5f * op, hm = 0.5f * (float)p.om_m;
-    const float usq = 1.5f * (ux * ux + uy * uy);
-    const float base = drho - rho * usq;   // E+/w without the 4.5 cu^2 term
-    // rest population
-    const float e0 = weight<float>(0) * base;
-    St::sth(dst, cell, h[0] - op * (h[0] - e0));
-    const float wr3[2] = {3.0f * weight<float>(1) * rho, 3.0f * weight<float>(5) * rho};
-    const float w45[2] = {4.5f * weight<float>(1) * rho, 4.5f * weight<float>(5) * rho};
-    const float wb[2] = {weight<float>(1) * base, weight<float>(5) * base};
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int q = 2 * k + 1, qb = q + 1;   // pairs (1,2) (3,4) (5,6) (7,8)
-        const int d = k < 2 ? 0 : 1;
-        const float cu = k == 0 ? ux : (k == 1 ? uy : (k == 2 ? ux + uy : uy - ux));
-        const float ep = wb[d] + w45[d] * (cu * cu);
-        const float em = wr3[d] * cu;
-        const float dp = hp * ((h[q] + h[qb]) - 2.0f * ep);
-        const float dm = hm * ((h[q] - h[qb]) - 2.0f * em);
-        St::sth(dst, (size_t)q * p.N + cell, h[q] - dp - dm);
-        St::sth(dst, (size_t)qb * p.N + cell, h[qb] - dp + dm);
-    }
+    for (int q = 0; q < 9; ++q) St::sth(dst, (size_t)q * p.N + cell, h[q]);
 }
 
 // Work-group index -> tile index, so that each XCD (work-groups are dealt
@@ -498,6 +520,7 @@ __device__ __forceinline__ void collide_any(const Params &p, int cell,
     }
 }
 
+// Initial and final passes, one node per work-item (grid = ceil(N/256/CPT)).
 template <int S, int MODE, int CPT>
 __global__ __launch_bounds__(256) void k_fused(Params p)
 {
@@ -505,7 +528,6 @@ __global__ __launch_bounds__(256) void k_fused(Params p)
     using T = typename St::T;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int base = tile * (256 * CPT) + threadIdx.x;
-    const bool link_tile = p.tile_slot && p.tile_slot[tile] >= 0;
     const int lx = p.nx - 1, ly = p.ny - 1;
     const int row = (MODE == MODE_INIT) ? 0 : bc_row(p);
     const bool macro = p.rho_out != nullptr;
@@ -524,44 +546,9 @@ __global__ __launch_bounds__(256) void k_fused(Params p)
         const int x = node_x(p, cell), y = cell - x * p.ny;
         const bool interior = (unsigned)(x - 1) < (unsigned)(lx - 1) &&
                               (unsigned)(y - 1) < (unsigned)(ly - 1);
-        if (MODE == MODE_STEP && interior) {
-            // interior node: unchecked pull; plane q is read at the constant
-            // node offset -(cx*ny + cy) from a work-group-uniform base
-            int lo = 0, hi = 0;
-            if (link_tile) link_range(p, cell, lo, hi);
-            if constexpr (S == LBM_STORE_F64) {
-#pragma unroll
-                for (int q = 0; q < 9; ++q) {
-                    const auto *pq = src + ((ptrdiff_t)q * p.N - (CX[q] * p.ny + CY[q]));
-                    f[q] = pq[cell];
-                }
-                if (lo < hi) {
-                    apply_links<S>(p, cell, lo, hi, f);
-                    store_drag_range<S>(p, cell, row, lo, hi, f);
-                }
-                collide_store<S>(p, cell, f, macro);
-            } else {
-                float h[9];
-#pragma unroll
-                for (int q = 0; q < 9; ++q) {
-                    const auto *pq = src + ((ptrdiff_t)q * p.N - (CX[q] * p.ny + CY[q]));
-                    h[q] = St::ldh(pq, cell);
-                }
-                if (lo < hi) {
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) f[q] = h[q] + weight<float>(q);
-                    apply_links<S>(p, cell, lo, hi, f);
-                    store_drag_range<S>(p, cell, row, lo, hi, f);
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) h[q] = f[q] - weight<float>(q);
-                }
-                collide_store_h<S>(p, cell, h, macro);
-            }
-            continue;
-        }
         T r = T(0), ux = T(0), uy = T(0);
         gather_bc<S>(p, x, y, cell, row, f, r, ux, uy);
-        if (link_tile) store_drag<S>(p, cell, row, f);
+        store_drag<S>(p, cell, row, f);
         if (MODE == MODE_STEP) {
             collide_any<S>(p, cell, f, macro);
         } else {  // MODE_EPILOGUE
@@ -670,6 +657,124 @@ __global__ __launch_bounds__(256) void k_step(Params p)
     }
 }
 
+// 16-bit storage forms, four consecutive nodes per work-item: every plane is
+// read and written with aligned 8-byte accesses.  A plane shifted by one node
+// in y (cy = +-1) takes its missing element from the neighbouring work-item
+// (cross-lane shuffle) or, at the edges of the wave's 256-node segment, from
+// one extra load.  Needs ny % 4 == 0 and N % (1024*CPT) == 0.
+__device__ __forceinline__ float h16_to_f(uint32_t b, bool bf16)
+{
+    if (bf16) return __uint_as_float(b << 16);
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)b);
+}
+
+__device__ __forceinline__ uint32_t f_to_h16(float v, bool bf16)
+{
+    if (bf16) return f32_to_bf16(v);
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v);
+}
+
+template <int S, int CPT>
+__global__ __launch_bounds__(256) void k_step_v4(Params p)
+{
+    constexpr bool BF = (S == LBM_STORE_BF16);
+    const int row = bc_row(p);
+    const bool macro = p.rho_out != nullptr;
+    if ((int)blockIdx.x >= p.n_main) {
+        using T = float;
+        const int i = ((int)blockIdx.x - p.n_main) * 256 + threadIdx.x;
+        if (i >= p.n_perim) return;
+        int x, y;
+        perim_node(p, i, x, y);
+        const int cell = x * p.ny + y;
+        T f[9], r = T(0), ux = T(0), uy = T(0);
+        gather_bc<S>(p, x, y, cell, row, f, r, ux, uy);
+        store_drag<S>(p, cell, row, f);
+        collide_any<S>(p, cell, f, macro);
+        return;
+    }
+    const uint16_t *__restrict__ src = static_cast<const uint16_t *>(p.src);
+    uint16_t *__restrict__ dst = static_cast<uint16_t *>(p.dst);
+    const int tile = xcd_tile(blockIdx.x, p.n_main);
+    const int slot = p.tile_slot ? p.tile_slot[tile] : -1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lx = p.nx - 1, ly = p.ny - 1;
+    const Coll kc = coll_consts(p);
+#pragma unroll 1
+    for (int c = 0; c < CPT; ++c) {
+        const int seg = tile * (1024 * CPT) + c * 1024 + wave * 256;  // wave's first node
+        const int cell0 = seg + lane * 4;
+        float h[4][9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const uint16_t *pq = src + ((ptrdiff_t)q * p.N - CX[q] * p.ny);
+            const uint2 a = *reinterpret_cast<const uint2 *>(pq + cell0);
+            uint32_t e[4] = {a.x & 0xffffu, a.x >> 16, a.y & 0xffffu, a.y >> 16};
+            if (CY[q] == 1) {          // source y-1: (prev, e0, e1, e2)
+                uint32_t prev = (uint32_t)__shfl_up((int)e[3], 1, 64);
+                const uint32_t edge = pq[seg - 1];
+                if (lane == 0) prev = edge;
+                e[3] = e[2]; e[2] = e[1]; e[1] = e[0]; e[0] = prev;
+            } else if (CY[q] == -1) {  // source y+1: (e1, e2, e3, next)
+                uint32_t next = (uint32_t)__shfl_down((int)e[0], 1, 64);
+                const uint32_t edge = pq[seg + 256];
+                if (lane == 63) next = edge;
+                e[0] = e[1]; e[1] = e[2]; e[2] = e[3]; e[3] = next;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) h[k][q] = h16_to_f(e[k], BF);
+        }
+        const int x = node_x(p, cell0), y0 = cell0 - x * p.ny;
+        const bool xin = (unsigned)(x - 1) < (unsigned)(lx - 1);
+        bool in[4];
+        uint32_t out[4][9];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int cell = cell0 + k;
+            in[k] = xin && (unsigned)(y0 + k - 1) < (unsigned)(ly - 1);
+            if (in[k] && slot >= 0) {
+                const int local = cell & (1024 * CPT - 1);
+                const uint32_t word = p.cell_mask[(size_t)slot * (32 * CPT) + (local >> 5)];
+                if ((word >> (local & 31)) & 1u) {
+                    int lo, hi;
+                    link_range(p, cell, lo, hi);
+                    float f[9];
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) f[q] = h[k][q] + weight<float>(q);
+                    apply_links<S>(p, cell, lo, hi, f);
+                    store_drag_range<S>(p, cell, row, lo, hi, f);
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) h[k][q] = f[q] - weight<float>(q);
+                }
+            }
+            float rho, ux, uy;
+            collide_h(kc, h[k], rho, ux, uy);
+            if (macro && in[k]) {
+                p.rho_out[cell] = (double)rho;
+                p.u_out[cell] = (double)ux;
+                p.u_out[(size_t)p.N + cell] = (double)uy;
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q) out[k][q] = f_to_h16(h[k][q], BF);
+        }
+        const bool all_in = in[0] && in[1] && in[2] && in[3];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            uint16_t *dq = dst + (size_t)q * p.N;
+            if (all_in) {
+                uint2 v;
+                v.x = out[0][q] | (out[1][q] << 16);
+                v.y = out[2][q] | (out[3][q] << 16);
+                *reinterpret_cast<uint2 *>(dq + cell0) = v;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (in[k]) dq[cell0 + k] = (uint16_t)out[k][q];
+            }
+        }
+    }
+}
+
 // Drag terms of one iteration on the link nodes only.
 template <int S>
 __global__ __launch_bounds__(256) void k_drag_only(Params p)
@@ -765,6 +870,7 @@ struct lbm_fused {
     int *d_tile_slot = nullptr, *d_cell_off = nullptr, *d_link_cells = nullptr;
     uint32_t *d_cell_mask = nullptr;
     int n_perim = 0;
+    bool vec4 = false;
     LinkRec *d_links = nullptr;
     int *d_obs_off = nullptr;
     int8_t *d_plan_q = nullptr;
@@ -840,13 +946,8 @@ int row_of(const lbm_fused *h, int64_t it)
 template <int S, int MODE>
 void launch_fused_s(const lbm_fused *h, const Params &p, hipStream_t s)
 {
-    const int grid = h->n_tiles;
-    switch (h->cpt) {
-    case 1: k_fused<S, MODE, 1><<<grid, 256, 0, s>>>(p); break;
-    case 2: k_fused<S, MODE, 2><<<grid, 256, 0, s>>>(p); break;
-    case 4: k_fused<S, MODE, 4><<<grid, 256, 0, s>>>(p); break;
-    default: k_fused<S, MODE, 8><<<grid, 256, 0, s>>>(p); break;
-    }
+    const int grid = (h->N + 511) / 512;
+    k_fused<S, MODE, 2><<<grid, 256, 0, s>>>(p);
 }
 
 template <int S> void launch_step_s(const lbm_fused *h, const Params &p, hipStream_t s)
@@ -860,8 +961,23 @@ template <int S> void launch_step_s(const lbm_fused *h, const Params &p, hipStre
     }
 }
 
+template <int S> void launch_step_v4(const lbm_fused *h, const Params &p, hipStream_t s)
+{
+    const int grid = h->n_tiles + (h->n_perim + 255) / 256;
+    switch (h->cpt) {
+    case 1: k_step_v4<S, 1><<<grid, 256, 0, s>>>(p); break;
+    case 2: k_step_v4<S, 2><<<grid, 256, 0, s>>>(p); break;
+    default: k_step_v4<S, 4><<<grid, 256, 0, s>>>(p); break;
+    }
+}
+
 void launch_step(const lbm_fused *h, const Params &p, hipStream_t s)
 {
+    if (h->vec4) {
+        if (h->storage == LBM_STORE_F16) launch_step_v4<LBM_STORE_F16>(h, p, s);
+        else launch_step_v4<LBM_STORE_BF16>(h, p, s);
+        return;
+    }
     switch (h->storage) {
     case LBM_STORE_F64: launch_step_s<LBM_STORE_F64>(h, p, s); break;
     case LBM_STORE_F32: launch_step_s<LBM_STORE_F32>(h, p, s); break;
@@ -1079,8 +1195,19 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     if (h->cpt == 0) h->cpt = (d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16) ? 4 : 2;
     if (h->cpt != 1 && h->cpt != 2 && h->cpt != 4 && h->cpt != 8)
         return set_error(LBM_ERR_ARG, "cells_per_thread must be 1, 2, 4 or 8");
-    h->tile_cells = 256 * h->cpt;
-    h->tile_shift = 8 + (h->cpt == 1 ? 0 : (h->cpt == 2 ? 1 : (h->cpt == 4 ? 2 : 3)));
+    // 16-bit forms use four nodes per work-item when the lattice allows it
+    const bool sixteen = d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16;
+    if (sixteen && d->ny % 4 == 0 && d->vector_width != 1) {
+        const int cpt = (h->cpt >= 4) ? 4 : h->cpt;
+        if ((long long)d->nx * d->ny % (1024LL * cpt) == 0) {
+            h->vec4 = true;
+            h->cpt = cpt;
+        }
+    }
+    const int per_thread = h->vec4 ? 4 * h->cpt : h->cpt;
+    h->tile_cells = 256 * per_thread;
+    h->tile_shift = 8 + (per_thread == 1 ? 0 : per_thread == 2 ? 1 : per_thread == 4 ? 2
+                         : per_thread == 8 ? 3 : 4);
     h->n_tiles = (h->N + h->tile_cells - 1) / h->tile_cells;
     h->n_perim = 2 * d->nx + 2 * d->ny - 4;
     h->magic = (uint32_t)((1ULL << 32) / (uint64_t)d->ny) + 1u;

@@ -14,7 +14,8 @@ from lbm_amd import _lib
 
 class FusedDomain:
     def __init__(self, nx, ny, om_p, om_m, bc, obstacles_links=(), ibb=False,
-                 obstacles_first=True, storage="f64", ring=1002, cells_per_thread=0):
+                 obstacles_first=True, storage="f64", ring=1002, cells_per_thread=0,
+                 vector_width=0):
         """bc: four LBM_BC_* codes (bottom, left, top, right).
         obstacles_links: sequence of (boundary int[nb,3], ibb float[nb] or None)."""
         self.L = _lib.load()
@@ -52,6 +53,7 @@ class FusedDomain:
         d.obs_offsets_host = self.obs_off.ctypes.data_as(ctypes.POINTER(_lib.c_i32))
         d.ring = self.ring
         d.cells_per_thread = int(cells_per_thread)
+        d.vector_width = int(vector_width)
         h = _lib.c_vp()
         _lib.check(self.L.lbm_fused_create(ctypes.byref(d), ctypes.byref(h)), "lbm_fused_create")
         self.h = h

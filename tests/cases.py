@@ -124,3 +124,90 @@ def run_pair(make, iterations, chunk=16, **kw):
     gpu = run_gpu(make, iterations, chunk=chunk, **kw)[0]
     ora = run_oracle(make, iterations, **kw)[0]
     return gpu, ora
+
+
+def array_domain_case(storage, iterations=80, vector_width=0, cpt=0, ny=64, n_obs=3, seed=1):
+    """Obstacle-array channel (nx = 4 ny) run directly through FusedDomain;
+    returns the final fields (epilogue), the drag series and the domain shape."""
+    import math
+    import types
+
+    import torch
+
+    from lbm_amd import _lib, geometry
+    from lbm_amd.app.configs import obstacle_array
+    from lbm_amd.domain import FusedDomain, pack_inlets
+    from lbm_amd.shapes import generate_shape
+    app = obstacle_array(ny=ny, aspect=4, n_obs=n_obs, seed=seed, precision=storage,
+                         iterations=iterations, sigma=20, n_spts=100, side_lo=6, side_hi=14)
+    lat = types.SimpleNamespace(nx=app.nx, ny=app.ny, x_min=app.x_min, x_max=app.x_max,
+                                y_min=app.y_min, y_max=app.y_max, dx=app.dx, IBB=True,
+                                lattice=np.zeros((app.nx, app.ny)))
+    for k, o in enumerate(app.obstacles):
+        s = generate_shape(o.n_pts, o.pos, o.type, o.size, o.name, o.n_spts, _TMP + "/")
+        o.set_polygon(s.curve_pts)
+        o.set_tag(k + 1)
+        o.fill(*geometry.add_obstacle(lat, o, verbose=False))
+    tau_p = app.tau_lbm
+    om_p, om_m = 1.0 / tau_p, 1.0 / (0.25 / (tau_p - 0.5) + 0.5)
+    nx, ny = app.nx, app.ny
+    bc = [_lib.BC_VELOCITY, _lib.BC_VELOCITY, _lib.BC_VELOCITY, _lib.BC_PRESSURE]
+    dom = FusedDomain(nx, ny, om_p, om_m, bc, [(o.boundary, o.ibb) for o in app.obstacles],
+                      ibb=True, storage=storage, ring=iterations + 2, cells_per_thread=cpt,
+                      vector_width=vector_width)
+    rows = []
+    for it in range(iterations + 1):
+        ret = (1.0 - math.exp(-it ** 2 / (2.0 * app.sigma ** 2)))
+        ul = np.zeros((2, ny))
+        ul[0, :] = ret * app.u_lbm
+        rows.append((np.zeros((2, nx)), ul, np.zeros((2, nx)), np.zeros((2, ny)),
+                     np.full(ny, 1.0)))
+    pool, offs = pack_inlets(rows)
+    dom.set_inlets(0, offs, pool)
+    w = torch.tensor([4. / 9.] + [1. / 9.] * 4 + [1. / 36.] * 4, dtype=torch.float64)
+    g0 = w[:, None, None].expand(9, nx, ny).contiguous().cuda()
+    rho = torch.empty((nx, ny), dtype=torch.float64, device="cuda")
+    u = torch.empty((2, nx, ny), dtype=torch.float64, device="cuda")
+    dom.init(g0)
+    for it in range(1, iterations - 1):
+        dom.step(it)
+    last = iterations - 1
+    dom.step(last, rho, u)
+    g = torch.empty((9, nx, ny), dtype=torch.float64, device="cuda")
+    gup = torch.empty_like(g)
+    dom.drag_only(last)
+    drag = dom.drag_read(0, iterations)
+    dom.epilogue(last, g, gup, rho, u)
+    out = dict(rho=rho.cpu().numpy(), u=u.cpu().numpy(), g=g.cpu().numpy(),
+               g_up=gup.cpu().numpy(), drag=drag, app=app, om=(om_p, om_m), rows=rows)
+    dom.close()
+    return out
+
+
+def array_oracle_case(gpu_out, iterations=80):
+    """The same run on the oracle, from the GPU case's geometry and inlets."""
+    from oracle import oracle
+    app = gpu_out["app"]
+    om_p, om_m = gpu_out["om"]
+    nx, ny = app.nx, app.ny
+    w = np.array([4. / 9.] + [1. / 9.] * 4 + [1. / 36.] * 4)
+    g = np.ascontiguousarray(np.broadcast_to(w[:, None, None], (9, nx, ny)))
+    geq, gup = np.zeros_like(g), np.zeros_like(g)
+    rho, u = np.ones((nx, ny)), np.zeros((2, nx, ny))
+    drag = []
+    for it in range(iterations):
+        ub, ul, ut, ur, rr = gpu_out["rows"][it]
+        oracle.macro(g, rho, u)
+        oracle.equilibrium(u, rho, geq)
+        oracle.col_str(g, geq, gup, om_p, om_m)
+        for o in app.obstacles:
+            oracle.bounce_back_obstacle(True, o.boundary, o.ibb, gup, g)
+        oracle.zou_he("bottom", u, rho, g, side=np.ascontiguousarray(ub))
+        oracle.zou_he("left", u, rho, g, side=np.ascontiguousarray(ul))
+        oracle.zou_he("top", u, rho, g, side=np.ascontiguousarray(ut))
+        oracle.zou_he("right_pressure", u, rho, g, side=np.ascontiguousarray(ur),
+                      rho_right=np.ascontiguousarray(rr))
+        for c in ("bottom_left", "top_left", "top_right", "bottom_right"):
+            oracle.zou_he(c, u, rho, g)
+        drag.append([oracle.drag_sums(o.boundary, gup, g) for o in app.obstacles])
+    return dict(rho=rho, u=u, g=g, g_up=gup, drag=np.array(drag))

@@ -1,0 +1,48 @@
+"""The four-node-per-work-item 16-bit step kernel (taken when ny % 4 == 0 and
+the node count is a multiple of 1024*CPT) against the one-node kernel of the
+same storage form -- same float arithmetic, so bit-identical fields and drag --
+and against the FP64 oracle within the 16-bit tolerances; the FP64 fused step
+on the same obstacle-array case is bit-identical to the oracle."""
+import numpy as np
+import pytest
+
+from tests.cases import array_domain_case, array_oracle_case
+
+pytestmark = pytest.mark.gpu
+
+IT = 80
+
+
+@pytest.fixture(scope="module")
+def oracle_run():
+    ref = array_domain_case("f64", IT)
+    return ref, array_oracle_case(ref, IT)
+
+
+def test_fp64_array_case_bit_identical(oracle_run):
+    g, o = oracle_run
+    for k in ("rho", "u", "g", "g_up"):
+        assert np.array_equal(g[k], o[k]), k
+    assert np.array_equal(g["drag"], o["drag"])
+
+
+@pytest.mark.parametrize("form", ["f16", "bf16"])
+@pytest.mark.parametrize("cpt", [1, 2, 4])
+def test_vector_kernel_matches_scalar(form, cpt):
+    print(f"case {form} cpt={cpt}", flush=True)
+    v = array_domain_case(form, IT, vector_width=4, cpt=cpt)
+    s = array_domain_case(form, IT, vector_width=1, cpt=cpt)
+    for k in ("rho", "u", "g", "g_up"):
+        assert np.array_equal(v[k], s[k]), k
+    assert np.array_equal(v["drag"], s["drag"])
+
+
+@pytest.mark.parametrize("form,tol", [("f16", 2e-2), ("bf16", 1e-1)])
+def test_vector_kernel_close_to_oracle(oracle_run, form, tol):
+    _, o = oracle_run
+    v = array_domain_case(form, IT, vector_width=4)
+    uref = np.abs(o["u"]).max()
+    assert np.abs(v["u"] - o["u"]).max() <= tol * uref
+    assert np.abs(v["rho"] - o["rho"]).max() <= tol / 10
+    dref = np.abs(o["drag"][..., 0]).max()
+    assert np.abs(v["drag"] - o["drag"]).max() <= tol * dref
