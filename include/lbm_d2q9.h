@@ -167,8 +167,8 @@ void lbm_fused_destroy(lbm_fused *h);
 int lbm_fused_set_inlets(lbm_fused *h, int row0, int nrows, const int32_t *offsets_host,
                          int pool_len, const double *pool_host, void *stream);
 
-/* Iteration 0: load g (double[9][nx][ny] on the device), take moments,
- * collide, store the post-collision field.  When rho_dev/u_dev are not
+/* Iteration 0: load g (double[9][nx][ny] on the device; NULL = the rest
+ * state g = w), take moments, collide, store the post-collision field.  When rho_dev/u_dev are not
  * NULL the moments of g (what lattice.macro() leaves) are stored there. */
 int lbm_init(lbm_fused *h, const double *g_dev, double *rho_dev, double *u_dev, void *stream);
 
@@ -205,6 +205,55 @@ int lbm_mass(lbm_fused *h, int64_t it, double *out_host, void *stream);
 
 /* Bytes of device memory held by the handle. */
 int64_t lbm_fused_device_bytes(const lbm_fused *h);
+
+/* ------------------------------------------------------------------------
+ * Batched ensembles (no counterpart in the reference, which runs one case per
+ * process): many lattices of different sizes, rates, walls and obstacles
+ * advanced together, one launch per iteration for the whole batch (a grid-
+ * stride sweep over every case's tiles), replayed from a HIP graph.  Each
+ * case is a full lbm_fused handle (lbm_batch_case) for per-case reads.
+ * Wall inputs are separable: the value of side s, element i, at iteration
+ * it is amp[it][case][s] * profile_s[i] -- the reference apps' inlets are of
+ * this form ((ret*u_lbm) * profile_j, cavity.py:65-76, turek.py:94-109).
+ * ---------------------------------------------------------------------- */
+typedef struct lbm_batch lbm_batch;
+
+typedef struct {
+    int32_t nx, ny;
+    double  om_p, om_m;
+    int32_t bc[4];                  /* as lbm_fused_desc.bc; corners always on */
+    const double *profile_host[5];  /* per side (LBM_SIDE_*): [2][nx|ny], rho_right [ny]; NULL = 0 */
+    int32_t n_links;                /* optional obstacle links, as lbm_fused_desc */
+    const int64_t *links_host;
+    const double  *ibb_host;
+    const uint8_t *link_ibb_host;
+    int32_t n_obs;
+    const int32_t *obs_offsets_host;
+} lbm_case_desc;
+
+typedef struct {
+    int32_t n_cases;
+    const lbm_case_desc *cases;
+    int32_t storage;                /* LBM_STORE_*, common to all cases */
+    int32_t cells_per_thread;       /* tile shape, common to all cases (0 = default) */
+    int32_t vector_width;           /* as lbm_fused_desc */
+    int32_t ring;                   /* rows of the amplitude table and drag rings */
+    int32_t grid;                   /* work-groups of the batched launch (0 = one per work item) */
+} lbm_batch_desc;
+
+int  lbm_batch_create(const lbm_batch_desc *desc, lbm_batch **out);
+void lbm_batch_destroy(lbm_batch *b);
+/* case c's handle (owned by the batch): lbm_step/lbm_epilogue/lbm_drag_read/... on one case */
+int  lbm_batch_case(lbm_batch *b, int c, lbm_fused **out);
+/* out4 = {work items, grid, cells per thread, vector width} */
+int  lbm_batch_info(const lbm_batch *b, int32_t *out4);
+/* amplitudes of iterations row0 .. row0+nrows-1: amp_host[nrows][n_cases][5] */
+int  lbm_batch_set_amplitudes(lbm_batch *b, int64_t row0, int nrows, const double *amp_host,
+                              void *stream);
+/* iteration 0 of every case from the rest state g = w */
+int  lbm_batch_init(lbm_batch *b, void *stream);
+/* iterations it0 .. it0+n-1 (it0 >= 1) of every case, graph-replayed */
+int  lbm_batch_steps(lbm_batch *b, int64_t it0, int n, void *stream);
 
 /* ------------------------------------------------------------------------
  * Measurement utility (no counterpart in the reference): device-to-device

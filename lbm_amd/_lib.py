@@ -44,6 +44,29 @@ class FusedDesc(ctypes.Structure):
     ]
 
 
+class CaseDesc(ctypes.Structure):
+    """Mirror of lbm_case_desc."""
+    _fields_ = [
+        ("nx", c_i32), ("ny", c_i32), ("om_p", c_dbl), ("om_m", c_dbl),
+        ("bc", c_i32 * 4),
+        ("profile_host", c_dp * 5),
+        ("n_links", c_i32),
+        ("links_host", ctypes.POINTER(c_i64)),
+        ("ibb_host", c_dp),
+        ("link_ibb_host", ctypes.POINTER(ctypes.c_uint8)),
+        ("n_obs", c_i32),
+        ("obs_offsets_host", ctypes.POINTER(c_i32)),
+    ]
+
+
+class BatchDesc(ctypes.Structure):
+    """Mirror of lbm_batch_desc."""
+    _fields_ = [
+        ("n_cases", c_i32), ("cases", ctypes.POINTER(CaseDesc)), ("storage", c_i32),
+        ("cells_per_thread", c_i32), ("vector_width", c_i32), ("ring", c_i32), ("grid", c_i32),
+    ]
+
+
 # (name, argtypes) of every exported entry point, in header order.
 _SIGNATURES = {
     "lbm_abi_version": ([], c_int),
@@ -75,6 +98,13 @@ _SIGNATURES = {
     "lbm_fused_restore": ([c_vp, c_i64, c_vp], c_int),
     "lbm_mass": ([c_vp, c_i64, c_vp, c_vp], c_int),
     "lbm_fused_device_bytes": ([c_vp], c_i64),
+    "lbm_batch_create": ([ctypes.POINTER(BatchDesc), ctypes.POINTER(c_vp)], c_int),
+    "lbm_batch_destroy": ([c_vp], None),
+    "lbm_batch_case": ([c_vp, c_int, ctypes.POINTER(c_vp)], c_int),
+    "lbm_batch_info": ([c_vp, c_vp], c_int),
+    "lbm_batch_set_amplitudes": ([c_vp, c_i64, c_int, c_vp, c_vp], c_int),
+    "lbm_batch_init": ([c_vp, c_vp], c_int),
+    "lbm_batch_steps": ([c_vp, c_i64, c_int, c_vp], c_int),
     "lbm_copy_probe": ([c_vp, c_vp, c_i64, c_int, c_vp], c_int),
 }
 

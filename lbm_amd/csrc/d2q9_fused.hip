@@ -132,6 +132,9 @@ struct Params {
     int bc_bottom, bc_left, bc_top, bc_right, corners;
     const double *pool;    // inlet side arrays
     const int *inlet_off;  // [ring][5] offsets into pool
+    const double *amp;     // separable inlets: amp[row * amp_stride + side] (or null)
+    int amp_stride;
+    const double *prof[5]; // separable inlets: per-side profiles
     int ring;
     const int *base;       // device int: row of iteration (it0-1) in graph replays
     int k;                 // row = (*base + k) mod ring  -- row of iteration it-1
@@ -294,6 +297,16 @@ __device__ __forceinline__ void apply_links(const Params &p, int cell, int lo, i
     }
 }
 
+// Wall input `side` of iteration row, element i (component-major, as the
+// reference's u_left[2][ny] etc.).  Pool form: a table of wall arrays per
+// iteration.  Separable form (batched cases): amplitude(row, side) x profile,
+// the reference apps' (ret*u_lbm) * profile_j values.
+__device__ __forceinline__ double side_value(const Params &p, int row, int side, int i)
+{
+    if (p.amp) return p.amp[(size_t)row * p.amp_stride + side] * p.prof[side][i];
+    return p.pool[p.inlet_off[row * 5 + side] + i];
+}
+
 // Post-boundary populations of a node that is not a domain corner; r, ux, uy
 // receive what the wall routine writes to rho/u (perimeter nodes only).
 template <int S>
@@ -306,34 +319,26 @@ __device__ void gather_edge(const Params &p, int x, int y, int cell, int row,
     int lo, hi;
     link_range(p, cell, lo, hi);
     apply_links<S>(p, cell, lo, hi, f);
-    const double *io = p.pool;
-    const int *off = p.inlet_off + row * 5;
     const int lx = p.nx - 1, ly = p.ny - 1;
     if (y == 0 && p.bc_bottom) {
-        const double *ub = io + off[LBM_SIDE_BOTTOM];
-        ux = T(ub[x]);
-        uy = T(ub[p.nx + x]);
+        ux = T(side_value(p, row, LBM_SIDE_BOTTOM, x));
+        uy = T(side_value(p, row, LBM_SIDE_BOTTOM, p.nx + x));
         zh_bottom(f, ux, uy, r);
     } else if (y == ly && p.bc_top) {
-        const double *ut = io + off[LBM_SIDE_TOP];
-        ux = T(ut[x]);
-        uy = T(ut[p.nx + x]);
+        ux = T(side_value(p, row, LBM_SIDE_TOP, x));
+        uy = T(side_value(p, row, LBM_SIDE_TOP, p.nx + x));
         zh_top(f, ux, uy, r);
     } else if (x == 0 && p.bc_left) {
-        const double *ul = io + off[LBM_SIDE_LEFT];
-        ux = T(ul[y]);
-        uy = T(ul[p.ny + y]);
+        ux = T(side_value(p, row, LBM_SIDE_LEFT, y));
+        uy = T(side_value(p, row, LBM_SIDE_LEFT, p.ny + y));
         zh_left(f, ux, uy, r);
     } else if (x == lx && p.bc_right == LBM_BC_VELOCITY) {
-        const double *ur = io + off[LBM_SIDE_RIGHT];
-        ux = T(ur[y]);
-        uy = T(ur[p.ny + y]);
+        ux = T(side_value(p, row, LBM_SIDE_RIGHT, y));
+        uy = T(side_value(p, row, LBM_SIDE_RIGHT, p.ny + y));
         zh_right_vel(f, ux, uy, r);
     } else if (x == lx && p.bc_right == LBM_BC_PRESSURE) {
-        const double *ur = io + off[LBM_SIDE_RIGHT];
-        const double *rr = io + off[LBM_SIDE_RHO_RIGHT];
-        r = T(rr[y]);
-        uy = T(ur[p.ny + y]);
+        r = T(side_value(p, row, LBM_SIDE_RHO_RIGHT, y));
+        uy = T(side_value(p, row, LBM_SIDE_RIGHT, p.ny + y));
         zh_right_pres(f, r, uy, ux);
     }
 }
@@ -538,8 +543,10 @@ __global__ __launch_bounds__(256) void k_fused(Params p)
         if (cell >= p.N) return;
         T f[9];
         if (MODE == MODE_INIT) {
+            // populations of iteration 0; without an array, the rest state g = w
 #pragma unroll
-            for (int q = 0; q < 9; ++q) f[q] = T(p.g0[(size_t)q * p.N + cell]);
+            for (int q = 0; q < 9; ++q)
+                f[q] = p.g0 ? T(p.g0[(size_t)q * p.N + cell]) : weight<T>(q);
             collide_any<S>(p, cell, f, macro);
             continue;
         }
@@ -586,28 +593,28 @@ __device__ __forceinline__ void perim_node(const Params &p, int i, int &x, int &
 // (perimeter nodes of the tile load a valid interior node and are skipped),
 // then fixes up obstacle-link nodes (tiles with links only) and collides.
 // Work-groups [n_main, grid) apply the Zou-He perimeter plan node by node.
+template <int S>
+__device__ __forceinline__ void step_perimeter(const Params &p, int pblock, int row, bool macro)
+{
+    using T = typename Store<S>::T;
+    const int i = pblock * 256 + threadIdx.x;
+    if (i >= p.n_perim) return;
+    int x, y;
+    perim_node(p, i, x, y);
+    const int cell = x * p.ny + y;
+    T f[9], r = T(0), ux = T(0), uy = T(0);
+    gather_bc<S>(p, x, y, cell, row, f, r, ux, uy);
+    store_drag<S>(p, cell, row, f);
+    collide_any<S>(p, cell, f, macro);
+}
+
 template <int S, int CPT>
-__global__ __launch_bounds__(256) void k_step(Params p)
+__device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bool macro)
 {
     using St = Store<S>;
     using T = typename St::T;
     using s_t = typename St::s_t;
-    const int row = bc_row(p);
-    const bool macro = p.rho_out != nullptr;
     const s_t *__restrict__ src = static_cast<const s_t *>(p.src);
-    if ((int)blockIdx.x >= p.n_main) {
-        const int i = ((int)blockIdx.x - p.n_main) * 256 + threadIdx.x;
-        if (i >= p.n_perim) return;
-        int x, y;
-        perim_node(p, i, x, y);
-        const int cell = x * p.ny + y;
-        T f[9], r = T(0), ux = T(0), uy = T(0);
-        gather_bc<S>(p, x, y, cell, row, f, r, ux, uy);
-        store_drag<S>(p, cell, row, f);
-        collide_any<S>(p, cell, f, macro);
-        return;
-    }
-    const int tile = xcd_tile(blockIdx.x, p.n_main);
     const int slot = p.tile_slot ? p.tile_slot[tile] : -1;
     const int base = tile * (256 * CPT) + threadIdx.x;
     const int lx = p.nx - 1, ly = p.ny - 1;
@@ -657,6 +664,22 @@ __global__ __launch_bounds__(256) void k_step(Params p)
     }
 }
 
+// The time step.  Work-groups [0, n_main) sweep interior tiles of 256*CPT
+// nodes: every work-item issues the 9*CPT pull loads of its nodes first
+// (perimeter nodes of the tile load a valid interior node and are skipped),
+// then fixes up obstacle-link nodes (tiles with links only) and collides.
+// Work-groups [n_main, grid) apply the Zou-He perimeter plan node by node.
+template <int S, int CPT>
+__global__ __launch_bounds__(256) void k_step(Params p)
+{
+    const int row = bc_row(p);
+    const bool macro = p.rho_out != nullptr;
+    if ((int)blockIdx.x >= p.n_main)
+        step_perimeter<S>(p, (int)blockIdx.x - p.n_main, row, macro);
+    else
+        step_tile<S, CPT>(p, xcd_tile(blockIdx.x, p.n_main), row, macro);
+}
+
 // 16-bit storage forms, four consecutive nodes per work-item: every plane is
 // read and written with aligned 8-byte accesses.  A plane shifted by one node
 // in y (cy = +-1) takes its missing element from the neighbouring work-item
@@ -675,27 +698,11 @@ __device__ __forceinline__ uint32_t f_to_h16(float v, bool bf16)
 }
 
 template <int S, int CPT>
-__global__ __launch_bounds__(256) void k_step_v4(Params p)
+__device__ __forceinline__ void step_tile_v4(const Params &p, int tile, int row, bool macro)
 {
     constexpr bool BF = (S == LBM_STORE_BF16);
-    const int row = bc_row(p);
-    const bool macro = p.rho_out != nullptr;
-    if ((int)blockIdx.x >= p.n_main) {
-        using T = float;
-        const int i = ((int)blockIdx.x - p.n_main) * 256 + threadIdx.x;
-        if (i >= p.n_perim) return;
-        int x, y;
-        perim_node(p, i, x, y);
-        const int cell = x * p.ny + y;
-        T f[9], r = T(0), ux = T(0), uy = T(0);
-        gather_bc<S>(p, x, y, cell, row, f, r, ux, uy);
-        store_drag<S>(p, cell, row, f);
-        collide_any<S>(p, cell, f, macro);
-        return;
-    }
     const uint16_t *__restrict__ src = static_cast<const uint16_t *>(p.src);
     uint16_t *__restrict__ dst = static_cast<uint16_t *>(p.dst);
-    const int tile = xcd_tile(blockIdx.x, p.n_main);
     const int slot = p.tile_slot ? p.tile_slot[tile] : -1;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lx = p.nx - 1, ly = p.ny - 1;
@@ -771,6 +778,43 @@ __global__ __launch_bounds__(256) void k_step_v4(Params p)
                 for (int k = 0; k < 4; ++k)
                     if (in[k]) dq[cell0 + k] = (uint16_t)out[k][q];
             }
+        }
+    }
+}
+
+template <int S, int CPT>
+__global__ __launch_bounds__(256) void k_step_v4(Params p)
+{
+    const int row = bc_row(p);
+    const bool macro = p.rho_out != nullptr;
+    if ((int)blockIdx.x >= p.n_main)
+        step_perimeter<S>(p, (int)blockIdx.x - p.n_main, row, macro);
+    else
+        step_tile_v4<S, CPT>(p, xcd_tile(blockIdx.x, p.n_main), row, macro);
+}
+
+// Batched time step: many lattices in one launch.  cases[2*c + parity] holds
+// the parameters of case c for iterations of that parity (ping-pong buffers);
+// work[] lists (case, block) items -- the interior tiles and perimeter blocks
+// of every case -- swept grid-stride by a grid sized to the GPU.
+template <int S, int CPT, bool V4>
+__global__ __launch_bounds__(256) void k_batch_step(const Params *__restrict__ cases,
+                                                    const int2 *__restrict__ work, int n_work,
+                                                    int k, const int *base, int parity)
+{
+    for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
+        const int2 item = work[w];
+        Params p = cases[2 * item.x + parity];
+        p.k = k;
+        p.base = base;
+        const int row = bc_row(p);
+        if (item.y >= p.n_main) {
+            step_perimeter<S>(p, item.y - p.n_main, row, false);
+        } else {
+            if constexpr (V4)
+                step_tile_v4<S, CPT>(p, item.y, row, false);
+            else
+                step_tile<S, CPT>(p, item.y, row, false);
         }
     }
 }
@@ -871,6 +915,9 @@ struct lbm_fused {
     uint32_t *d_cell_mask = nullptr;
     int n_perim = 0;
     bool vec4 = false;
+    const double *amp = nullptr;   // separable inlets (batched cases)
+    int amp_stride = 0;
+    const double *prof[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     LinkRec *d_links = nullptr;
     int *d_obs_off = nullptr;
     int8_t *d_plan_q = nullptr;
@@ -920,6 +967,9 @@ Params base_params(const lbm_fused *h)
     p.corners = h->corners;
     p.pool = h->d_pool;
     p.inlet_off = h->d_inlet_off;
+    p.amp = h->amp;
+    p.amp_stride = h->amp_stride;
+    for (int i = 0; i < 5; ++i) p.prof[i] = h->prof[i];
     p.ring = h->ring;
     p.base = h->d_zero;
     p.k = 0;
@@ -1304,7 +1354,7 @@ extern "C" int lbm_fused_set_inlets(lbm_fused *h, int row0, int nrows, const int
 
 extern "C" int lbm_init(lbm_fused *h, const double *g, double *rho, double *u, void *stream)
 {
-    if (!h || !g || ((rho == nullptr) != (u == nullptr)))
+    if (!h || ((rho == nullptr) != (u == nullptr)))
         return set_error(LBM_ERR_ARG, "lbm_init: bad argument");
     Params p = base_params(h);
     p.g0 = g;
@@ -1320,7 +1370,7 @@ extern "C" int lbm_step(lbm_fused *h, int64_t it, double *rho, double *u, void *
 {
     if (!h || it < 1 || ((rho == nullptr) != (u == nullptr)))
         return set_error(LBM_ERR_ARG, "lbm_step: bad argument");
-    if (!h->d_pool) return set_error(LBM_ERR_ARG, "lbm_step: no inlet table");
+    if (!h->d_pool && !h->amp) return set_error(LBM_ERR_ARG, "lbm_step: no inlet table");
     Params p = base_params(h);
     p.src = h->buf[(it - 1) & 1];
     p.dst = h->buf[it & 1];
@@ -1336,7 +1386,7 @@ extern "C" int lbm_steps(lbm_fused *h, int64_t it0, int n, void *stream)
 {
     if (!h || it0 < 1 || n < 0) return set_error(LBM_ERR_ARG, "lbm_steps: bad argument");
     if (n == 0) return LBM_OK;
-    if (!h->d_pool) return set_error(LBM_ERR_ARG, "lbm_steps: no inlet table");
+    if (!h->d_pool && !h->amp) return set_error(LBM_ERR_ARG, "lbm_steps: no inlet table");
     const GraphKey key{n, (int)(it0 & 1)};
     auto itg = h->graphs.find(key);
     if (itg == h->graphs.end()) {
@@ -1371,7 +1421,7 @@ extern "C" int lbm_epilogue(lbm_fused *h, int64_t it, double *g, double *gup, do
 {
     if (!h || it < 0 || !g || !gup || ((rho == nullptr) != (u == nullptr)))
         return set_error(LBM_ERR_ARG, "lbm_epilogue: bad argument");
-    if (!h->d_pool) return set_error(LBM_ERR_ARG, "lbm_epilogue: no inlet table");
+    if (!h->d_pool && !h->amp) return set_error(LBM_ERR_ARG, "lbm_epilogue: no inlet table");
     Params p = base_params(h);
     p.src = h->buf[it & 1];
     p.g_out = g;
@@ -1387,7 +1437,7 @@ extern "C" int lbm_epilogue(lbm_fused *h, int64_t it, double *g, double *gup, do
 extern "C" int lbm_drag_only(lbm_fused *h, int64_t it, void *stream)
 {
     if (!h || it < 0) return set_error(LBM_ERR_ARG, "lbm_drag_only: bad argument");
-    if (!h->d_pool) return set_error(LBM_ERR_ARG, "lbm_drag_only: no inlet table");
+    if (!h->d_pool && !h->amp) return set_error(LBM_ERR_ARG, "lbm_drag_only: no inlet table");
     Params p = base_params(h);
     p.src = h->buf[it & 1];
     p.k = row_of(h, it);
@@ -1465,5 +1515,257 @@ extern "C" int lbm_mass(lbm_fused *h, int64_t it, double *out, void *stream)
     LBM_HIP_TRY(hipMemcpyAsync(out, h->d_part + nb, sizeof(double), hipMemcpyDeviceToHost,
                                S(stream)));
     LBM_HIP_TRY(hipStreamSynchronize(S(stream)));
+    return LBM_OK;
+}
+
+// ================================================================= batch ===
+struct lbm_batch {
+    std::vector<lbm_fused *> cases;
+    std::vector<double *> prof;          // 5 per case
+    Params *d_params = nullptr;          // [n_cases][2]
+    int2 *d_work = nullptr;
+    int n_work = 0;
+    double *d_amp = nullptr;             // [ring][n_cases][5]
+    int ring = 0, storage = 0, cpt = 1, grid = 0;
+    bool vec4 = false;
+    int *d_base = nullptr;
+    hipStream_t cap_stream = nullptr;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+};
+
+namespace {
+
+void batch_free(lbm_batch *b)
+{
+    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+    for (lbm_fused *h : b->cases) lbm_fused_destroy(h);
+    for (double *p : b->prof)
+        if (p) (void)hipFree(p);
+    void *ptrs[] = {b->d_params, b->d_work, b->d_amp, b->d_base};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+}
+
+template <int S, int CPT, bool V4>
+void launch_batch_s(const lbm_batch *b, int k, int parity, hipStream_t s)
+{
+    k_batch_step<S, CPT, V4><<<b->grid, 256, 0, s>>>(b->d_params, b->d_work, b->n_work, k,
+                                                      b->d_base, parity);
+}
+
+template <int S> void launch_batch_t(const lbm_batch *b, int k, int parity, hipStream_t s)
+{
+    if constexpr (S == LBM_STORE_F16 || S == LBM_STORE_BF16) {
+        if (b->vec4) {
+            switch (b->cpt) {
+            case 1: launch_batch_s<S, 1, true>(b, k, parity, s); return;
+            case 2: launch_batch_s<S, 2, true>(b, k, parity, s); return;
+            default: launch_batch_s<S, 4, true>(b, k, parity, s); return;
+            }
+        }
+    }
+    switch (b->cpt) {
+    case 1: launch_batch_s<S, 1, false>(b, k, parity, s); break;
+    case 2: launch_batch_s<S, 2, false>(b, k, parity, s); break;
+    case 4: launch_batch_s<S, 4, false>(b, k, parity, s); break;
+    default: launch_batch_s<S, 8, false>(b, k, parity, s); break;
+    }
+}
+
+void launch_batch(const lbm_batch *b, int k, int parity, hipStream_t s)
+{
+    switch (b->storage) {
+    case LBM_STORE_F64: launch_batch_t<LBM_STORE_F64>(b, k, parity, s); break;
+    case LBM_STORE_F32: launch_batch_t<LBM_STORE_F32>(b, k, parity, s); break;
+    case LBM_STORE_F16: launch_batch_t<LBM_STORE_F16>(b, k, parity, s); break;
+    default: launch_batch_t<LBM_STORE_BF16>(b, k, parity, s); break;
+    }
+}
+
+}  // namespace
+
+extern "C" int lbm_batch_create(const lbm_batch_desc *d, lbm_batch **out)
+{
+    if (!d || !out || d->n_cases <= 0 || !d->cases || d->ring < 2)
+        return set_error(LBM_ERR_ARG, "lbm_batch_create: bad argument");
+    *out = nullptr;
+    auto b = std::make_unique<lbm_batch>();
+    b->ring = d->ring;
+    b->storage = d->storage;
+    const int n = d->n_cases;
+    // one vector width for every case: four nodes per work-item only if all allow it
+    int vw = d->vector_width;
+    if (vw != 1 && (d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16)) {
+        const int cpt = d->cells_per_thread >= 4 || d->cells_per_thread == 0 ? 4 : d->cells_per_thread;
+        for (int c = 0; c < n; ++c) {
+            const lbm_case_desc &cd = d->cases[c];
+            if (cd.ny % 4 || ((long long)cd.nx * cd.ny) % (1024LL * cpt)) vw = 1;
+        }
+    }
+    if (hipMalloc((void **)&b->d_amp, sizeof(double) * (size_t)b->ring * n * 5) != hipSuccess ||
+        hipMemset(b->d_amp, 0, sizeof(double) * (size_t)b->ring * n * 5) != hipSuccess) {
+        batch_free(b.get());
+        return set_error(LBM_ERR_HIP, "batch amplitude table");
+    }
+    std::vector<Params> params;
+    std::vector<int2> work;
+    for (int c = 0; c < n; ++c) {
+        const lbm_case_desc &cd = d->cases[c];
+        lbm_fused_desc fd{};
+        fd.nx = cd.nx;
+        fd.ny = cd.ny;
+        fd.storage = d->storage;
+        fd.om_p = cd.om_p;
+        fd.om_m = cd.om_m;
+        for (int i = 0; i < 4; ++i) fd.bc[i] = cd.bc[i];
+        fd.corners = 1;
+        fd.obstacles_first = 1;
+        fd.n_links = cd.n_links;
+        fd.links_host = cd.links_host;
+        fd.ibb_host = cd.ibb_host;
+        fd.link_ibb_host = cd.link_ibb_host;
+        fd.n_obs = cd.n_obs;
+        fd.obs_offsets_host = cd.obs_offsets_host;
+        fd.ring = d->ring;
+        fd.cells_per_thread = d->cells_per_thread;
+        fd.vector_width = vw;
+        lbm_fused *h = nullptr;
+        if (int e = lbm_fused_create(&fd, &h)) {
+            batch_free(b.get());
+            return set_error(e, "case " + std::to_string(c) + ": " + g_last_error);
+        }
+        b->cases.push_back(h);
+        if (c == 0) {
+            b->cpt = h->cpt;
+            b->vec4 = h->vec4;
+        } else if (h->cpt != b->cpt || h->vec4 != b->vec4) {
+            batch_free(b.get());
+            return set_error(LBM_ERR_ARG, "cases resolve to different tile shapes");
+        }
+        // wall profiles: u_bot, u_left, u_top, u_right ([2][len]) and rho_right ([ny])
+        const int len[5] = {2 * cd.nx, 2 * cd.ny, 2 * cd.nx, 2 * cd.ny, cd.ny};
+        for (int sd = 0; sd < 5; ++sd) {
+            double *dp = nullptr;
+            if (hipMalloc((void **)&dp, sizeof(double) * len[sd]) != hipSuccess) {
+                batch_free(b.get());
+                return set_error(LBM_ERR_HIP, "profile allocation");
+            }
+            b->prof.push_back(dp);
+            std::vector<double> hv(len[sd], 0.0);
+            if (cd.profile_host[sd]) std::memcpy(hv.data(), cd.profile_host[sd], sizeof(double) * len[sd]);
+            if (hipMemcpy(dp, hv.data(), sizeof(double) * len[sd], hipMemcpyHostToDevice) != hipSuccess) {
+                batch_free(b.get());
+                return set_error(LBM_ERR_HIP, "profile upload");
+            }
+            h->prof[sd] = dp;
+        }
+        h->amp = b->d_amp + (size_t)c * 5;
+        h->amp_stride = n * 5;
+        for (int parity = 0; parity < 2; ++parity) {
+            Params p = base_params(h);
+            p.src = h->buf[parity ^ 1];   // iteration of this parity reads the other buffer
+            p.dst = h->buf[parity];
+            p.rho_out = nullptr;
+            p.u_out = nullptr;
+            params.push_back(p);
+        }
+        const int blocks = h->n_tiles + (h->n_perim + 255) / 256;
+        for (int k = 0; k < blocks; ++k) work.push_back(make_int2(c, k));
+    }
+    b->n_work = (int)work.size();
+    if (hipMalloc((void **)&b->d_params, sizeof(Params) * params.size()) != hipSuccess ||
+        hipMalloc((void **)&b->d_work, sizeof(int2) * work.size()) != hipSuccess ||
+        hipMalloc((void **)&b->d_base, sizeof(int)) != hipSuccess ||
+        hipMemcpy(b->d_params, params.data(), sizeof(Params) * params.size(),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_work, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+        batch_free(b.get());
+        return set_error(LBM_ERR_HIP, "batch tables");
+    }
+    b->grid = d->grid > 0 ? std::min(d->grid, b->n_work) : b->n_work;
+    *out = b.release();
+    return LBM_OK;
+}
+
+extern "C" void lbm_batch_destroy(lbm_batch *b)
+{
+    if (!b) return;
+    (void)hipDeviceSynchronize();
+    batch_free(b);
+    delete b;
+}
+
+extern "C" int lbm_batch_case(lbm_batch *b, int c, lbm_fused **out)
+{
+    if (!b || !out || c < 0 || c >= (int)b->cases.size())
+        return set_error(LBM_ERR_ARG, "lbm_batch_case: bad argument");
+    *out = b->cases[c];
+    return LBM_OK;
+}
+
+extern "C" int lbm_batch_info(const lbm_batch *b, int32_t *out4)
+{
+    if (!b || !out4) return set_error(LBM_ERR_ARG, "lbm_batch_info: bad argument");
+    out4[0] = b->n_work;
+    out4[1] = b->grid;
+    out4[2] = b->cpt;
+    out4[3] = b->vec4 ? 4 : 1;
+    return LBM_OK;
+}
+
+extern "C" int lbm_batch_set_amplitudes(lbm_batch *b, int64_t row0, int nrows,
+                                        const double *amp_host, void *stream)
+{
+    if (!b || nrows < 0 || nrows > b->ring || (nrows && !amp_host))
+        return set_error(LBM_ERR_ARG, "lbm_batch_set_amplitudes: bad argument");
+    LBM_HIP_TRY(hipStreamSynchronize(S(stream)));
+    const size_t per_row = b->cases.size() * 5;
+    int r = 0;
+    while (r < nrows) {
+        const int row = (int)(((row0 + r) % b->ring + b->ring) % b->ring);
+        const int len = std::min(nrows - r, b->ring - row);
+        LBM_HIP_TRY(hipMemcpy(b->d_amp + (size_t)row * per_row, amp_host + (size_t)r * per_row,
+                              sizeof(double) * per_row * len, hipMemcpyHostToDevice));
+        r += len;
+    }
+    return LBM_OK;
+}
+
+extern "C" int lbm_batch_init(lbm_batch *b, void *stream)
+{
+    if (!b) return set_error(LBM_ERR_ARG, "lbm_batch_init: bad argument");
+    for (lbm_fused *h : b->cases)
+        if (int e = lbm_init(h, nullptr, nullptr, nullptr, stream)) return e;
+    return LBM_OK;
+}
+
+extern "C" int lbm_batch_steps(lbm_batch *b, int64_t it0, int n, void *stream)
+{
+    if (!b || it0 < 1 || n < 0) return set_error(LBM_ERR_ARG, "lbm_batch_steps: bad argument");
+    if (n == 0) return LBM_OK;
+    const GraphKey key{n, (int)(it0 & 1)};
+    auto itg = b->graphs.find(key);
+    if (itg == b->graphs.end()) {
+        hipGraph_t graph;
+        LBM_HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < n; ++k) launch_batch(b, k, (int)((it0 + k) & 1), b->cap_stream);
+        hipError_t ec = hipStreamEndCapture(b->cap_stream, &graph);
+        if (ec != hipSuccess)
+            return set_error(LBM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+        hipGraphExec_t exec;
+        ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ec != hipSuccess)
+            return set_error(LBM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ec));
+        itg = b->graphs.emplace(key, exec).first;
+    }
+    const int base = (int)(((it0 - 1) % b->ring + b->ring) % b->ring);
+    k_set_int<<<1, 64, 0, S(stream)>>>(b->d_base, base);
+    LBM_LAUNCH_CHECK();
+    LBM_HIP_TRY(hipGraphLaunch(itg->second, S(stream)));
     return LBM_OK;
 }

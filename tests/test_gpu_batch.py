@@ -1,0 +1,132 @@
+"""Batched ensembles (lbm_batch_*): every case of a batch advanced by the
+batched launch equals the same case run alone through the single-domain
+fused step -- bit for bit in FP64 -- and the FP32 batch stays within the
+FP32 tolerance of the FP64 runs.  Cases differ in size (odd sizes included),
+relaxation rates and walls; one carries IBB obstacle links."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from lbm_amd import _lib
+from lbm_amd.batch import Batch, autotune, cavity_amplitudes, cavity_ensemble
+from lbm_amd.domain import FusedDomain, pack_inlets
+
+pytestmark = pytest.mark.gpu
+
+IT = 60
+
+
+def _cases():
+    cases = cavity_ensemble(n_cases=5, seed=3, n_lo=17, n_hi=50)
+    for c in cases:
+        c["sigma"] = 15
+    # a channel with a 5x5 block (plain links listed by hand, IBB p = 0.3)
+    nx, ny = 64, 24
+    solid = np.zeros((nx, ny), bool)
+    solid[20:25, 9:14] = True
+    C = np.array([[0, 0], [1, 0], [-1, 0], [0, 1], [0, -1], [1, 1], [-1, -1], [-1, 1], [1, -1]])
+    links = np.array([[i, j, q] for i in range(1, nx - 1) for j in range(1, ny - 1)
+                      if not solid[i, j] for q in range(1, 9)
+                      if solid[i + C[q, 0], j + C[q, 1]]], dtype=np.int64)
+    left = np.zeros((2, ny))
+    yy = np.arange(ny)
+    left[0] = 4.0 * yy * (ny - 1 - yy) / (ny - 1) ** 2
+    tau = 0.6
+    cases.append(dict(nx=nx, ny=ny, om_p=1 / tau, om_m=1 / (0.25 / (tau - 0.5) + 0.5),
+                      bc=[_lib.BC_VELOCITY, _lib.BC_VELOCITY, _lib.BC_VELOCITY, _lib.BC_PRESSURE],
+                      profiles=[None, left, None, None, np.ones(ny)], links=links,
+                      ibb=np.full(len(links), 0.3), ibb_on=np.ones(len(links), np.uint8),
+                      obs_offsets=[0, len(links)], sigma=15, u_lid=0.05, channel=True))
+    return cases
+
+
+def _amp(cases, n):
+    amp = np.zeros((n, len(cases), 5))
+    for k in range(n):
+        for c, cs in enumerate(cases):
+            ret = (1.0 - math.exp(-k ** 2 / (2.0 * cs["sigma"] ** 2)))
+            if cs.get("channel"):
+                amp[k, c, 1] = ret * cs["u_lid"]
+                amp[k, c, 4] = 1.0
+            else:
+                amp[k, c, 2] = cs["u_lid"] * ret
+    return amp
+
+
+def _final(dom, nx, ny, last):
+    rho = torch.empty((nx, ny), dtype=torch.float64, device="cuda")
+    u = torch.empty((2, nx, ny), dtype=torch.float64, device="cuda")
+    g = torch.empty((9, nx, ny), dtype=torch.float64, device="cuda")
+    gup = torch.empty_like(g)
+    dom.step(last, rho, u)
+    dom.epilogue(last, g, gup, rho, u)
+    return dict(rho=rho.cpu().numpy(), u=u.cpu().numpy(), g=g.cpu().numpy(),
+                g_up=gup.cpu().numpy())
+
+
+def _single(cs, amp_c, storage):
+    """One case alone through FusedDomain with a pool inlet table of the same values."""
+    nx, ny = cs["nx"], cs["ny"]
+    links = cs.get("links")
+    obs = [(links, cs["ibb"])] if links is not None else []
+    dom = FusedDomain(nx, ny, cs["om_p"], cs["om_m"], cs["bc"], obs, ibb=links is not None,
+                      storage=storage, ring=IT + 2)
+    rows = []
+    profs = cs["profiles"]
+    for k in range(IT + 1):
+        snap = []
+        for sd, n in enumerate((2 * nx, 2 * ny, 2 * nx, 2 * ny, ny)):
+            shape = (ny,) if sd == 4 else (2, n // 2)
+            a = np.zeros(shape) if profs[sd] is None else amp_c[k, sd] * np.asarray(profs[sd])
+            snap.append(a)
+        rows.append(tuple(snap))
+    pool, offs = pack_inlets(rows)
+    dom.set_inlets(0, offs, pool)
+    dom.init(None)
+    for it in range(1, IT - 1):
+        dom.step(it)
+    out = _final(dom, nx, ny, IT - 1)
+    dom.close()
+    return out
+
+
+def _batch(cases, amp, storage, **kw):
+    b = Batch(cases, storage=storage, ring=IT + 2, **kw)
+    b.set_amplitudes(0, amp[:IT + 1])
+    b.init()
+    b.steps(1, IT - 2)
+    outs = [_final(b.case(c), cs["nx"], cs["ny"], IT - 1) for c, cs in enumerate(cases)]
+    b.close()
+    return outs
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(grid=7), dict(cells_per_thread=1, grid=3)])
+def test_batch_fp64_equals_single_runs(kw):
+    cases = _cases()
+    amp = _amp(cases, IT + 1)
+    outs = _batch(cases, amp, "f64", **kw)
+    for c, cs in enumerate(cases):
+        ref = _single(cs, amp[:, c], "f64")
+        for k in ("rho", "u", "g", "g_up"):
+            assert np.array_equal(outs[c][k], ref[k]), (c, k)
+
+
+def test_batch_fp32_close_to_fp64():
+    cases = _cases()
+    amp = _amp(cases, IT + 1)
+    outs = _batch(cases, amp, "f32")
+    for c, cs in enumerate(cases):
+        ref = _single(cs, amp[:, c], "f64")
+        uref = np.abs(ref["u"]).max()
+        assert np.abs(outs[c]["u"] - ref["u"]).max() <= 2e-5 * uref
+        assert np.abs(outs[c]["rho"] - ref["rho"]).max() <= 2e-6
+
+
+def test_autotune_returns_a_valid_shape(tmp_path):
+    cases = _cases()
+    best, res = autotune(cases, "f32", steps=4, cache=str(tmp_path / "t.json"))
+    assert len(res) >= 3 and best["cells_per_thread"] in (1, 2, 4, 8)
+    b = Batch(cases, storage="f32", ring=8, **best)
+    b.close()
