@@ -256,6 +256,46 @@ int  lbm_batch_init(lbm_batch *b, void *stream);
 int  lbm_batch_steps(lbm_batch *b, int64_t it0, int n, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Drag sensitivity by a discrete adjoint (no counterpart in the reference).
+ * A channel with the channel apps' boundary plan (Zou-He walls at rest,
+ * velocity inlet u = a(it) * profile on the left with a(it) = ramp(it) * u_max,
+ * pressure outlet rho_out, the four corners; turek.py:94-125) and an obstacle
+ * given by a per-node solid fraction s in [0,1] (partial bounce-back:
+ * post-collision H = (1-s) c + s h_opposite).  The loss is the mean over the
+ * last `window` iterations of the momentum-exchange drag
+ * D = sum_x s sum_q c_qx (c_q - h_opposite).  lbm_pbb_forward keeps the state
+ * every `checkpoint_every` iterations; lbm_pbb_backward recomputes each
+ * segment from its checkpoint and runs the hand-written adjoint step
+ * backwards, giving dL/ds (every node) and dL/du_max.  Storage: LBM_STORE_F64
+ * or LBM_STORE_F32 (states and arithmetic); drag sums and gradient
+ * accumulators are double.
+ * ---------------------------------------------------------------------- */
+typedef struct lbm_pbb lbm_pbb;
+
+typedef struct {
+    int32_t nx, ny;
+    int32_t storage;                 /* LBM_STORE_F64 or LBM_STORE_F32 */
+    double  om_p, om_m;              /* TRT rates */
+    double  rho_out;                 /* outlet density */
+    const double *inlet_profile_host;/* [2][ny] inlet velocity of unit amplitude */
+    const double *ramp_host;         /* [iterations+1] ramp(it) */
+    int32_t iterations;              /* T: iterations 1..T after the start from rest */
+    int32_t window;                  /* the loss averages D over iterations T-window+1..T */
+    int32_t checkpoint_every;
+} lbm_pbb_desc;
+
+int  lbm_pbb_create(const lbm_pbb_desc *desc, lbm_pbb **out);
+void lbm_pbb_destroy(lbm_pbb *h);
+/* s_dev: [nx][ny] solid fraction in the storage type; writes the loss */
+int  lbm_pbb_forward(lbm_pbb *h, const void *s_dev, double u_max, double *loss_host, void *stream);
+/* after lbm_pbb_forward: s_grad_dev [nx][ny] double, *umax_grad_host */
+int  lbm_pbb_backward(lbm_pbb *h, double *s_grad_dev, double *umax_grad_host, void *stream);
+/* drag D of iterations 0..T of the last forward run: out_host[T+1] */
+int  lbm_pbb_drag_history(const lbm_pbb *h, double *out_host);
+/* post-collision deviations H_T = f - w of the last forward run ([9][nx][ny], storage type) */
+int  lbm_pbb_final_state(lbm_pbb *h, void *dst_dev, void *stream);
+
+/* ------------------------------------------------------------------------
  * Measurement utility (no counterpart in the reference): device-to-device
  * copy of `bytes` (a multiple of 16) with `blocks` work-groups of 256
  * work-items and 16-byte accesses -- the HBM ceiling the time step is

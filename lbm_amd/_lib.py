@@ -67,6 +67,15 @@ class BatchDesc(ctypes.Structure):
     ]
 
 
+class PbbDesc(ctypes.Structure):
+    """Mirror of lbm_pbb_desc."""
+    _fields_ = [
+        ("nx", c_i32), ("ny", c_i32), ("storage", c_i32), ("om_p", c_dbl), ("om_m", c_dbl),
+        ("rho_out", c_dbl), ("inlet_profile_host", c_dp), ("ramp_host", c_dp),
+        ("iterations", c_i32), ("window", c_i32), ("checkpoint_every", c_i32),
+    ]
+
+
 # (name, argtypes) of every exported entry point, in header order.
 _SIGNATURES = {
     "lbm_abi_version": ([], c_int),
@@ -105,6 +114,12 @@ _SIGNATURES = {
     "lbm_batch_set_amplitudes": ([c_vp, c_i64, c_int, c_vp, c_vp], c_int),
     "lbm_batch_init": ([c_vp, c_vp], c_int),
     "lbm_batch_steps": ([c_vp, c_i64, c_int, c_vp], c_int),
+    "lbm_pbb_create": ([ctypes.POINTER(PbbDesc), ctypes.POINTER(c_vp)], c_int),
+    "lbm_pbb_destroy": ([c_vp], None),
+    "lbm_pbb_forward": ([c_vp, c_vp, c_dbl, c_vp, c_vp], c_int),
+    "lbm_pbb_backward": ([c_vp, c_vp, c_vp, c_vp], c_int),
+    "lbm_pbb_drag_history": ([c_vp, c_vp], c_int),
+    "lbm_pbb_final_state": ([c_vp, c_vp, c_vp], c_int),
     "lbm_copy_probe": ([c_vp, c_vp, c_i64, c_int, c_vp], c_int),
 }
 
