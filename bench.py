@@ -185,6 +185,239 @@ def cpu_baseline(app, lat, om_p, om_m, iterations):
     return nx * ny * iterations / dt / 1e6, dt
 
 
+def turek_case(ny=200):
+    """Schaefer-Turek channel (turek200 app): geometry, rates, inlet profile."""
+    from lbm_amd import geometry
+    from lbm_amd.app.configs import turek200
+    from lbm_amd.shapes import generate_shape
+    app = turek200(iterations=200000)
+    app.L_lbm = ny
+    app.compute_lbm_parameters()
+    app.obstacles[0].size = app.r_cyl
+    app.it_max = 199999
+    lat = types.SimpleNamespace(nx=app.nx, ny=app.ny, x_min=app.x_min, x_max=app.x_max,
+                                y_min=app.y_min, y_max=app.y_max, dx=app.dx, IBB=True,
+                                lattice=np.zeros((app.nx, app.ny)))
+    o = app.obstacles[0]
+    sh = generate_shape(o.n_pts, o.pos, o.type, o.size, o.name, o.n_spts,
+                        "/tmp/lbm_amd_bench_shapes/")
+    o.set_polygon(sh.curve_pts)
+    o.set_tag(1)
+    o.fill(*geometry.add_obstacle(lat, o, verbose=False))
+    prof = np.zeros((2, app.ny))
+    for j in range(app.ny):
+        prof[:, j] = app.poiseuille(lat_coords(app, 0, j))
+    return app, o, prof
+
+
+def lat_coords(app, i, j):
+    dx = (app.x_max - app.x_min) / (app.nx - 1)
+    dy = (app.y_max - app.y_min) / (app.ny - 1)
+    return [app.x_min + i * dx, app.y_min + j * dy]
+
+
+def run_turek(app, o, prof, storage, iterations):
+    """Turek channel through a one-case batch (separable inlet); returns the
+    per-iteration Cd/Cl (reference convention), the final velocity field and
+    the timed MLUPS."""
+    import torch
+    from lbm_amd import _lib
+    from lbm_amd.batch import Batch
+    tau_m = 0.25 / (app.tau_lbm - 0.5) + 0.5
+    case = dict(nx=app.nx, ny=app.ny, om_p=1.0 / app.tau_lbm, om_m=1.0 / tau_m,
+                bc=[_lib.BC_VELOCITY, _lib.BC_VELOCITY, _lib.BC_VELOCITY, _lib.BC_PRESSURE],
+                profiles=[None, prof, None, None, np.ones(app.ny)], links=o.boundary,
+                ibb=o.ibb, ibb_on=np.ones(len(o.boundary), np.uint8),
+                obs_offsets=[0, len(o.boundary)])
+    b = Batch([case], storage=storage, ring=iterations + 2)
+    amp = np.zeros((iterations + 1, 1, 5))
+    for it in range(iterations + 1):
+        ret = (1.0 - math.exp(-it ** 2 / (2.0 * app.sigma ** 2)))
+        amp[it, 0, 1] = ret * app.u_lbm
+        amp[it, 0, 4] = app.rho_lbm
+    b.set_amplitudes(0, amp)
+    b.init()
+    torch.cuda.synchronize()
+    s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s0.record()
+    it = 1
+    last = iterations - 1
+    while it < last:
+        n = min(1000, last - it)
+        b.steps(it, n)
+        it += n
+    s1.record()
+    torch.cuda.synchronize()
+    ms = s0.elapsed_time(s1)
+    dom = b.case(0)
+    rho = torch.empty((app.nx, app.ny), dtype=torch.float64, device="cuda")
+    u = torch.empty((2, app.nx, app.ny), dtype=torch.float64, device="cuda")
+    g = torch.empty((9, app.nx, app.ny), dtype=torch.float64, device="cuda")
+    gup = torch.empty_like(g)
+    dom.step(last, rho, u)
+    dom.drag_only(last)
+    fxy = dom.drag_read(0, iterations)[:, 0, :]
+    dom.epilogue(last, g, gup, rho, u)
+    ux = u.cpu().numpy()
+    b.close()
+    scale = -2.0 / (app.rho_lbm * app.D_lbm * app.u_avg ** 2)
+    return dict(cd=fxy[:, 0] * scale, cl=fxy[:, 1] * scale, u=ux,
+                mlups=app.nx * app.ny * (last - 1) / (ms * 1e-3) / 1e6)
+
+
+def shedding_stats(app, cd, cl, tail=50000):
+    """max |Cd|, max |Cl| and the Strouhal number f D / U over the last `tail`
+    iterations (period from mean-crossings of the lift)."""
+    c = cl[-tail:] - cl[-tail:].mean()
+    up = np.nonzero((c[:-1] < 0) & (c[1:] >= 0))[0]
+    st = None
+    if len(up) >= 3:
+        period = (up[-1] - up[0]) / (len(up) - 1)
+        st = app.D_lbm / (period * app.u_avg)
+    return dict(max_cd=float(np.abs(cd[-tail:]).max()), max_cl=float(np.abs(cl[-tail:]).max()),
+                strouhal=st)
+
+
+def section_turek(forms, iterations, log):
+    app, o, prof = turek_case()
+    out = {}
+    ref = None
+    for f in forms:
+        log(f"# turek200 {f}: {iterations} iterations")
+        r = run_turek(app, o, prof, f, iterations)
+        st = shedding_stats(app, r["cd"], r["cl"], tail=min(50000, iterations // 2))
+        entry = dict(mlups=round(r["mlups"], 1), **st)
+        if f == "f64":
+            ref = (r, st)
+        elif ref is not None:
+            r64, st64 = ref
+            entry["max_cd_rel_err"] = abs(st["max_cd"] - st64["max_cd"]) / st64["max_cd"]
+            entry["max_cl_rel_err"] = abs(st["max_cl"] - st64["max_cl"]) / st64["max_cl"]
+            if st["strouhal"] and st64["strouhal"]:
+                entry["strouhal_rel_err"] = abs(st["strouhal"] - st64["strouhal"]) / st64["strouhal"]
+            entry["u_l2_rel_err"] = float(np.linalg.norm(r["u"] - r64["u"]) /
+                                          np.linalg.norm(r64["u"]))
+        out[f] = entry
+        log(f"# turek200 {f}: {entry}")
+    out["config"] = "turek200: 1073x200, IBB cylinder D=48, u_max=0.05, Re=100, %d iterations" \
+        % iterations
+    return out
+
+
+def section_cavity(iterations, log):
+    """Lid-driven cavity 200x200, FP64 (launch-bound: 40 000 nodes)."""
+    import torch
+    from lbm_amd.batch import Batch, cavity_amplitudes, cavity_ensemble
+    cases = cavity_ensemble(n_cases=1, seed=0, n_lo=200, n_hi=200, re_lo=100.0, re_hi=100.0)
+    b = Batch(cases, storage="f64", ring=iterations + 2)
+    b.set_amplitudes(0, cavity_amplitudes(cases, 0, iterations + 1))
+    b.init()
+    torch.cuda.synchronize()
+    s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s0.record()
+    it = 1
+    while it < iterations:
+        n = min(1000, iterations - it)
+        b.steps(it, n)
+        it += n
+    s1.record()
+    torch.cuda.synchronize()
+    ms = s0.elapsed_time(s1)
+    b.close()
+    out = dict(config="cavity 200x200, u_lid=0.1, Re=100, FP64, %d iterations" % iterations,
+               mlups=round(200 * 200 * (iterations - 1) / (ms * 1e-3) / 1e6, 1),
+               us_per_iteration=round(ms * 1e3 / (iterations - 1), 2))
+    log(f"# cavity200: {out}")
+    return out
+
+
+def section_batch(iterations, log):
+    """256 seeded cavities (n ~ U{64..512}, Re ~ logU[100,1000]) in one launch
+    per iteration, FP32, autotuned; per-case error vs the FP64 solves."""
+    import torch
+    from lbm_amd.batch import Batch, autotune, cavity_amplitudes, cavity_ensemble
+    cases = cavity_ensemble(n_cases=256, seed=0)
+    best, tried = autotune(cases, "f32", steps=20, cache="/tmp/lbm_amd_autotune.json", log=log)
+    amp = cavity_amplitudes(cases, 0, iterations + 1)
+    res = {}
+    for storage, kw in (("f32", best), ("f64", {})):
+        b = Batch(cases, storage=storage, ring=iterations + 2, **kw)
+        b.set_amplitudes(0, amp)
+        b.init()
+        torch.cuda.synchronize()
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s0.record()
+        it = 1
+        last = iterations - 1
+        while it < last:
+            n = min(1000, last - it)
+            b.steps(it, n)
+            it += n
+        s1.record()
+        torch.cuda.synchronize()
+        ms = s0.elapsed_time(s1)
+        us = []
+        for c, cs in enumerate(cases):
+            n = cs["nx"]
+            rho = torch.empty((n, n), dtype=torch.float64, device="cuda")
+            u = torch.empty((2, n, n), dtype=torch.float64, device="cuda")
+            b.case(c).step(last, rho, u)
+            us.append(u.cpu().numpy())
+        res[storage] = dict(ms=ms, u=us, mlups=b.cells * (last - 1) / (ms * 1e-3) / 1e6,
+                            grid=b.grid, cpt=b.cpt)
+        b.close()
+        log(f"# batch {storage}: {res[storage]['mlups']:.0f} MLUPS")
+    errs = [float(np.linalg.norm(a - r) / np.linalg.norm(r))
+            for a, r in zip(res["f32"]["u"], res["f64"]["u"])]
+    cells = int(sum(c["nx"] * c["ny"] for c in cases))
+    return dict(config="256 cavities, n~U{64..512}, Re~logU[100,1000], u_lid=0.1, %d iterations, "
+                       "one launch per iteration" % iterations,
+                cells=cells, f32_mlups=round(res["f32"]["mlups"], 1),
+                f32_hbm_frac=round(res["f32"]["mlups"] * 1e6 * 72 / 8e12, 4),
+                f64_mlups=round(res["f64"]["mlups"], 1),
+                autotuned=dict(best, candidates=len(tried)),
+                u_l2_rel_err_vs_f64=dict(max=max(errs), median=float(np.median(errs))))
+
+
+def section_adjoint(log):
+    """C5: 1024x256 channel, solid-fraction disk + noise, 2000 iterations,
+    loss over the last 1000, checkpoints every 100; FP32 forward/adjoint."""
+    import torch
+    from lbm_amd.adjoint import PbbEngine, cd_scale, cylinder_case
+    case = cylinder_case()
+    out = {"config": "1024x256 channel, disk D=40 at (200,128) as solid fraction 1, U[0,0.2] "
+                     "noise in a 60x60 box, parabolic inlet u_max=0.05, Re=100, 2000 "
+                     "iterations, loss = mean drag of the last 1000, checkpoint every 100"}
+    grads = {}
+    for storage in ("f64", "f32"):
+        eng = PbbEngine(case, storage=storage)
+        dt = torch.float64 if storage == "f64" else torch.float32
+        s = torch.tensor(case["s"], dtype=dt, device="cuda")
+        eng.forward(s, case["u_max"])           # warm-up
+        eng.backward()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = eng.forward(s, case["u_max"])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        gs, gu = eng.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        grads[storage] = (loss, gs.cpu().numpy(), gu)
+        out[storage] = dict(loss=loss, cd=loss * cd_scale(case), forward_ms=round((t1 - t0) * 1e3, 2),
+                            backward_ms=round((t2 - t1) * 1e3, 2),
+                            adjoint_to_forward=round((t2 - t1) / (t1 - t0), 3),
+                            dL_du_max=gu)
+        eng.close()
+        log(f"# adjoint {storage}: {out[storage]}")
+    l64, g64, u64 = grads["f64"]
+    l32, g32, u32 = grads["f32"]
+    out["f32_vs_f64"] = dict(loss_rel_err=abs(l32 - l64) / abs(l64),
+                             grad_s_rel_l2=float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+                             grad_umax_rel_err=abs(u32 - u64) / abs(u64))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +429,10 @@ def main():
     ap.add_argument("--ny", type=int, default=2048)
     ap.add_argument("--cpu-iterations", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sections", default="c3,c1,c2,c4,c5",
+                    help="c3 obstacle array (headline), c1 cavity, c2 turek, c4 batch, c5 adjoint")
+    ap.add_argument("--turek-iterations", type=int, default=200000)
+    ap.add_argument("--batch-iterations", type=int, default=5000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -256,6 +493,20 @@ def main():
                          f"workload, FP64, oracle/d2q9_oracle.c driven in the reference's call "
                          f"order ({secs:.1f} s)"}
     copy_gbs = copy_ceiling_gbs()
+    sections = set(x.strip() for x in args.sections.split(","))
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    extra = {}
+    if "c1" in sections:
+        extra["cavity"] = section_cavity(20001, log)
+    if "c2" in sections:
+        extra["turek"] = section_turek(["f64", "f32", "f16"], args.turek_iterations, log)
+    if "c4" in sections:
+        extra["batch"] = section_batch(args.batch_iterations, log)
+    if "c5" in sections:
+        extra["adjoint"] = section_adjoint(log)
     line = {
         "metric": "D2Q9 MLUPS (FP16-offset storage, 8192x2048 obstacle array) and % of HBM roofline",
         "value": round(h["mlups"], 1),
@@ -282,6 +533,7 @@ def main():
         "cpu_baseline": cpu,
         "forms": forms_out,
     }
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 
