@@ -1219,10 +1219,6 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
         free_all(h.get());
         return set_error(LBM_ERR_HIP, "hipMemset failed");
     }
-    if (hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) != hipSuccess) {
-        free_all(h.get());
-        return set_error(LBM_ERR_HIP, "hipStreamCreate failed");
-    }
     *out = h.release();
     return LBM_OK;
 }
@@ -1311,6 +1307,8 @@ extern "C" int lbm_steps(lbm_fused *h, int64_t it0, int n, void *stream)
     const GraphKey key{n, (int)(it0 & 1)};
     auto itg = h->graphs.find(key);
     if (itg == h->graphs.end()) {
+        if (!h->cap_stream)   // created on first use: batched cases never capture
+            LBM_HIP_TRY(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
         hipGraph_t graph;
         LBM_HIP_TRY(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
         for (int k = 0; k < n; ++k) {
