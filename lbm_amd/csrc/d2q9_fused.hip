@@ -195,6 +195,25 @@ __device__ __forceinline__ void pull_checked(const Params &p, int x, int y,
     }
 }
 
+// Deviation value h_qb of one link in the float forms.  The two or three
+// sources of a link share the weight w_q (planes q and qb are opposite) and
+// the interpolation factors sum to one, so f - w obeys the same affine map:
+// the rest state bounces back exactly and float-rounded factors cost a
+// relative 1e-7 of the deviation, not of w.
+template <int S>
+__device__ __forceinline__ float link_h(const Params &p, int cell, const LinkRec &L)
+{
+    using St = Store<S>;
+    const auto *src = static_cast<const typename St::s_t *>(p.src);
+    const float A = St::ldh(src, (size_t)L.q * p.N + cell);
+    if (!L.mode) return A;
+    const float B = St::ldh(src, (size_t)L.planeB * p.N + L.cellB);
+    const float C = St::ldh(src, (size_t)L.planeC * p.N + L.cellC);
+    return ((float)L.c0 * A + (float)L.c1 * B) + (float)L.c2 * C;
+}
+
+// Obstacle links of a node on full populations f (nb.py:77-117): f[qb] of
+// every link of `cell` (links lo..hi-1).
 template <int S>
 __device__ __forceinline__ void apply_links(const Params &p, int cell, int lo, int hi,
                                             typename Store<S>::T *f)
@@ -204,17 +223,36 @@ __device__ __forceinline__ void apply_links(const Params &p, int cell, int lo, i
     const auto *src = static_cast<const typename St::s_t *>(p.src);
     for (int k = lo; k < hi; ++k) {
         const LinkRec L = p.links[k];
-        const T A = St::ld(src, (size_t)L.q * p.N + cell, L.q);
-        T v = A;
-        if (L.mode) {
-            const T B = St::ld(src, (size_t)L.planeB * p.N + L.cellB, L.planeB);
-            const T C = St::ld(src, (size_t)L.planeC * p.N + L.cellC, L.planeC);
-            v = (T(L.c0) * A + T(L.c1) * B) + T(L.c2) * C;
+        T v;
+        if constexpr (S == LBM_STORE_F64) {
+            const T A = St::ld(src, (size_t)L.q * p.N + cell, L.q);
+            v = A;
+            if (L.mode) {
+                const T B = St::ld(src, (size_t)L.planeB * p.N + L.cellB, L.planeB);
+                const T C = St::ld(src, (size_t)L.planeC * p.N + L.cellC, L.planeC);
+                v = (T(L.c0) * A + T(L.c1) * B) + T(L.c2) * C;
+            }
+        } else {
+            v = weight<float>(L.qb) + link_h<S>(p, cell, L);
         }
         // f[L.qb] = v with a register-resident array
 #pragma unroll
         for (int q = 1; q < 9; ++q)
             if (q == L.qb) f[q] = v;
+    }
+}
+
+// The same on deviations h (float forms, interior nodes).
+template <int S>
+__device__ __forceinline__ void apply_links_h(const Params &p, int cell, int lo, int hi,
+                                              float *h)
+{
+    for (int k = lo; k < hi; ++k) {
+        const LinkRec L = p.links[k];
+        const float v = link_h<S>(p, cell, L);
+#pragma unroll
+        for (int q = 1; q < 9; ++q)
+            if (q == L.qb) h[q] = v;
     }
 }
 
@@ -290,6 +328,10 @@ __device__ void gather_bc(const Params &p, int x, int y, int cell, int row,
     zh_corner(f, which, r, ux, uy);
 }
 
+// Drag ring entry of every link of `cell`: f_q(x) + f_qb(x), the pair the
+// momentum exchange sums (nb.py:49-73), in double.  The float forms add the
+// exact double weights to the deviations, so the weights cancel over an
+// obstacle as they do in FP64.
 template <int S>
 __device__ __forceinline__ void store_drag_range(const Params &p, int cell, int row, int lo,
                                                  int hi, const typename Store<S>::T *f)
@@ -298,12 +340,42 @@ __device__ __forceinline__ void store_drag_range(const Params &p, int cell, int 
     const auto *src = static_cast<const typename St::s_t *>(p.src);
     for (int k = lo; k < hi; ++k) {
         const LinkRec L = p.links[k];
-        double fb = 0.0;
+        double v;
+        if constexpr (S == LBM_STORE_F64) {
+            double fb = 0.0;
+#pragma unroll
+            for (int q = 1; q < 9; ++q)
+                if (q == L.qb) fb = f[q];
+            v = St::ld(src, (size_t)L.q * p.N + cell, L.q) + fb;
+        } else {
+            // f[qb] = fl(w + h) with |h| < w/2, so f[qb] - w is exact (Sterbenz)
+            float hb = 0.0f;
+#pragma unroll
+            for (int q = 1; q < 9; ++q)
+                if (q == L.qb) hb = f[q] - weight<float>(q);
+            const double hA = (double)St::ldh(src, (size_t)L.q * p.N + cell);
+            v = 2.0 * weight<double>(L.q) + (hA + (double)hb);
+        }
+        p.drag_ring[(size_t)row * p.n_links + L.dslot] = v;
+    }
+}
+
+// The same from deviations h (float forms, interior nodes).
+template <int S>
+__device__ __forceinline__ void store_drag_range_h(const Params &p, int cell, int row, int lo,
+                                                   int hi, const float *h)
+{
+    using St = Store<S>;
+    const auto *src = static_cast<const typename St::s_t *>(p.src);
+    for (int k = lo; k < hi; ++k) {
+        const LinkRec L = p.links[k];
+        float hb = 0.0f;
 #pragma unroll
         for (int q = 1; q < 9; ++q)
-            if (q == L.qb) fb = (double)f[q];
-        const double A = (double)St::ld(src, (size_t)L.q * p.N + cell, L.q);
-        p.drag_ring[(size_t)row * p.n_links + L.dslot] = A + fb;
+            if (q == L.qb) hb = h[q];
+        const double hA = (double)St::ldh(src, (size_t)L.q * p.N + cell);
+        p.drag_ring[(size_t)row * p.n_links + L.dslot] =
+            2.0 * weight<double>(L.q) + (hA + (double)hb);
     }
 }
 
@@ -368,10 +440,12 @@ __device__ __forceinline__ Coll coll_consts(const Params &p)
 }
 
 // In place: h (deviations of one node) -> post-collision deviations.
+// Every multiply-add is an explicit fmaf and nothing else is contracted (the
+// library builds with -ffp-contract=off), so each kernel that inlines this
+// -- one node per lane, four nodes per lane, batched -- rounds identically.
 __device__ __forceinline__ void collide_h(const Coll &k, float *h, float &rho, float &ux,
                                           float &uy)
 {
-#pragma clang fp contract(fast)
     const float drho = ((h[0] + h[1]) + (h[2] + h[3])) + ((h[4] + h[5]) + (h[6] + h[7])) + h[8];
     const float jx = (h[1] - h[2]) + (h[5] - h[6]) + (h[8] - h[7]);
     const float jy = (h[3] - h[4]) + (h[5] - h[6]) + (h[7] - h[8]);
@@ -379,10 +453,10 @@ __device__ __forceinline__ void collide_h(const Coll &k, float *h, float &rho, f
     const float inv = __builtin_amdgcn_rcpf(rho);
     ux = jx * inv;
     uy = jy * inv;
-    const float usq = 1.5f * (ux * ux + uy * uy);
-    const float base = drho - rho * usq;   // E+/w without the 4.5 cu^2 term
+    const float usq = 1.5f * __builtin_fmaf(uy, uy, ux * ux);
+    const float base = __builtin_fmaf(-rho, usq, drho);   // E+/w without the 4.5 cu^2 term
     const float e0 = weight<float>(0) * base;
-    h[0] = h[0] - k.op * (h[0] - e0);
+    h[0] = __builtin_fmaf(-k.op, h[0] - e0, h[0]);
     const float wr3[2] = {3.0f * weight<float>(1) * rho, 3.0f * weight<float>(5) * rho};
     const float w45[2] = {4.5f * weight<float>(1) * rho, 4.5f * weight<float>(5) * rho};
     const float wb[2] = {weight<float>(1) * base, weight<float>(5) * base};
@@ -391,12 +465,12 @@ __device__ __forceinline__ void collide_h(const Coll &k, float *h, float &rho, f
         const int q = 2 * j + 1, qb = q + 1;   // pairs (1,2) (3,4) (5,6) (7,8)
         const int d = j < 2 ? 0 : 1;
         const float cu = j == 0 ? ux : (j == 1 ? uy : (j == 2 ? ux + uy : uy - ux));
-        const float ep = wb[d] + w45[d] * (cu * cu);
+        const float ep = __builtin_fmaf(w45[d] * cu, cu, wb[d]);
         const float em = wr3[d] * cu;
-        const float dp = k.hp * ((h[q] + h[qb]) - 2.0f * ep);
-        const float dm = k.hm * ((h[q] - h[qb]) - 2.0f * em);
-        const float a = h[q] - dp - dm;
-        const float bb = h[qb] - dp + dm;
+        const float dp = k.hp * __builtin_fmaf(-2.0f, ep, h[q] + h[qb]);
+        const float dm = k.hm * __builtin_fmaf(-2.0f, em, h[q] - h[qb]);
+        const float a = (h[q] - dp) - dm;
+        const float bb = (h[qb] - dp) + dm;
         h[q] = a;
         h[qb] = bb;
     }
@@ -567,15 +641,13 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
             if ((word >> (local & 31)) & 1u) {
                 int lo, hi;
                 link_range(p, cell, lo, hi);
-                T f[9];
-#pragma unroll
-                for (int q = 0; q < 9; ++q)
-                    f[q] = (S == LBM_STORE_F64) ? v[c][q] : v[c][q] + weight<float>(q);
-                apply_links<S>(p, cell, lo, hi, f);
-                store_drag_range<S>(p, cell, row, lo, hi, f);
-#pragma unroll
-                for (int q = 0; q < 9; ++q)
-                    v[c][q] = (S == LBM_STORE_F64) ? f[q] : f[q] - weight<float>(q);
+                if constexpr (S == LBM_STORE_F64) {
+                    apply_links<S>(p, cell, lo, hi, v[c]);
+                    store_drag_range<S>(p, cell, row, lo, hi, v[c]);
+                } else {
+                    apply_links_h<S>(p, cell, lo, hi, v[c]);
+                    store_drag_range_h<S>(p, cell, row, lo, hi, v[c]);
+                }
             }
         }
         if constexpr (S == LBM_STORE_F64)
@@ -666,13 +738,8 @@ __device__ __forceinline__ void step_tile_v4(const Params &p, int tile, int row,
                 if ((word >> (local & 31)) & 1u) {
                     int lo, hi;
                     link_range(p, cell, lo, hi);
-                    float f[9];
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) f[q] = h[k][q] + weight<float>(q);
-                    apply_links<S>(p, cell, lo, hi, f);
-                    store_drag_range<S>(p, cell, row, lo, hi, f);
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) h[k][q] = f[q] - weight<float>(q);
+                    apply_links_h<S>(p, cell, lo, hi, h[k]);
+                    store_drag_range_h<S>(p, cell, row, lo, hi, h[k]);
                 }
             }
             float rho, ux, uy;

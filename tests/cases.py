@@ -211,3 +211,34 @@ def array_oracle_case(gpu_out, iterations=80):
             oracle.zou_he(c, u, rho, g)
         drag.append([oracle.drag_sums(o.boundary, gup, g) for o in app.obstacles])
     return dict(rho=rho, u=u, g=g, g_up=gup, drag=np.array(drag))
+
+
+def one_step_pair(storage="f16", cpt=1, ny=64, nx=256, seed=5, iterations=2):
+    """Two FusedDomains of the same channel (no obstacles), 4-node and 1-node
+    16-bit kernels, advanced from one random state; returns the g_up of the
+    last iteration of each (decoded to double)."""
+    import torch
+    from lbm_amd import _lib
+    from lbm_amd.domain import FusedDomain, pack_inlets
+    rng = np.random.default_rng(seed)
+    w = np.array([4. / 9.] + [1. / 9.] * 4 + [1. / 36.] * 4)
+    g0 = w[:, None, None] * (1.0 + 0.05 * rng.standard_normal((9, nx, ny)))
+    g0 = torch.from_numpy(np.ascontiguousarray(g0)).cuda()
+    bc = [_lib.BC_VELOCITY, _lib.BC_VELOCITY, _lib.BC_VELOCITY, _lib.BC_PRESSURE]
+    rows = [(np.zeros((2, nx)), np.zeros((2, ny)), np.zeros((2, nx)), np.zeros((2, ny)),
+             np.ones(ny))] * (iterations + 1)
+    pool, offs = pack_inlets(rows)
+    outs = []
+    for vw in (4, 1):
+        dom = FusedDomain(nx, ny, 1 / 0.6, 1 / (0.25 / 0.1 + 0.5), bc, storage=storage,
+                          ring=iterations + 2, cells_per_thread=cpt, vector_width=vw)
+        dom.set_inlets(0, offs, pool)
+        dom.init(g0)
+        for it in range(1, iterations + 1):
+            dom.step(it)
+        g = torch.empty((9, nx, ny), dtype=torch.float64, device="cuda")
+        gup = torch.empty_like(g)
+        dom.epilogue(iterations, g, gup)
+        outs.append(gup.cpu().numpy())
+        dom.close()
+    return outs

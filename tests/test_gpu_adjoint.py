@@ -6,7 +6,9 @@
 * FP64 adjoint gradients (dL/ds on every node, dL/du_max) equal torch
   autograd of that restatement within 1e-9 relative to the largest entry.
 * Central differences of the FP64 GPU forward on 16 seeded nodes and u_max
-  agree with the adjoint within 1e-5 relative (C5 geometry, full size).
+  agree with the adjoint within 1e-5 relative (plus 1e-6 of the largest
+  sampled gradient, the FP64 round-off floor of the differences) on the
+  1024x256 channel at full size.
 * The gradient does not depend on the checkpoint interval (bit-identical).
 * The FP32 adjoint stays within 2e-2 (relative L2) of the FP64 adjoint.
 """
@@ -89,30 +91,41 @@ def test_autograd_function():
 
 
 def test_central_differences_full_size():
-    """C5 geometry (1024x256, disk D=40 at (200,128), noisy 60x60 box, 2000
-    iterations, loss over the last 1000), FP64: 16 seeded nodes in the box and
-    u_max, central differences against the adjoint."""
+    """Channel 1024x256, disk D=40 at (200,128) in a noisy 60x60 box, 2000
+    iterations, loss over the last 1000, FP64: 16 seeded nodes of the box
+    outside the disk and u_max, central differences against the adjoint."""
     case = cylinder_case()
     eng = PbbEngine(case, storage="f64")
     s = torch.tensor(case["s"], dtype=torch.float64, device="cuda")
     eng.forward(s, case["u_max"])
     gs, gu = eng.backward()
     gs = gs.cpu().numpy()
+    # nodes of the noisy box outside the disk (inside it s = 1 and the
+    # gradient is exactly zero)
+    sc = np.asarray(case["s"])
+    cand = np.argwhere((sc > 0.0) & (sc < 1.0))
     rng = np.random.default_rng(7)
-    nodes = [(int(rng.integers(172, 229)), int(rng.integers(100, 157))) for _ in range(16)]
-    eps = 1e-4
-    worst = 0.0
+    nodes = [tuple(int(v) for v in cand[k]) for k in rng.choice(len(cand), 16, replace=False)]
+    # central differences: truncation O(eps^2); FP64 round-off of a loss summed
+    # over 1000 iterations, divided by 2 eps, leaves ~1e-11 absolute noise, so
+    # nodes with near-zero gradient are held to 1e-6 of the largest one
+    eps = 1e-5
+    rows = []
     for (i, j) in nodes:
         sp = s.clone()
         sp[i, j] += eps
         sm = s.clone()
         sm[i, j] -= eps
         fd = (eng.forward(sp, case["u_max"]) - eng.forward(sm, case["u_max"])) / (2 * eps)
-        ref = max(abs(fd), 1e-3 * np.abs(gs).max())
-        worst = max(worst, abs(fd - gs[i, j]) / ref)
-    assert worst <= 1e-5, worst
-    e = 1e-6
+        rows.append((i, j, fd, float(gs[i, j])))
+        print(f"node ({i},{j}) s={float(s[i, j]):.4f} fd={fd:.10e} adjoint={gs[i, j]:.10e}",
+              flush=True)
+    scale = max(abs(r[3]) for r in rows)
+    for (i, j, fd, ad) in rows:
+        assert abs(fd - ad) <= 1e-5 * abs(ad) + 1e-6 * scale, (i, j, fd, ad)
+    e = 1e-7
     fd = (eng.forward(s, case["u_max"] + e) - eng.forward(s, case["u_max"] - e)) / (2 * e)
+    print(f"u_max fd={fd:.10e} adjoint={gu:.10e}", flush=True)
     assert abs(fd - gu) <= 1e-5 * abs(fd)
     eng.close()
 

@@ -6,11 +6,12 @@ on the same obstacle-array case is bit-identical to the oracle."""
 import numpy as np
 import pytest
 
-from tests.cases import array_domain_case, array_oracle_case
+from tests.cases import array_domain_case, array_oracle_case, one_step_pair
 
 pytestmark = pytest.mark.gpu
 
 IT = 80
+IT_LONG = 400
 
 
 @pytest.fixture(scope="module")
@@ -37,12 +38,33 @@ def test_vector_kernel_matches_scalar(form, cpt):
     assert np.array_equal(v["drag"], s["drag"])
 
 
+@pytest.fixture(scope="module")
+def oracle_long():
+    """400 iterations: the inflow has reached the obstacles (drag ~0.3, not
+    round-off).  At tau = 0.5038 the vortices leaving through the pressure
+    outlet amplify any rounding after ~500 iterations -- FP32 departs from
+    FP64 there too -- so the comparison stops before."""
+    ref = array_domain_case("f64", IT_LONG)
+    return array_oracle_case(ref, IT_LONG)
+
+
 @pytest.mark.parametrize("form,tol", [("f16", 2e-2), ("bf16", 1e-1)])
-def test_vector_kernel_close_to_oracle(oracle_run, form, tol):
-    _, o = oracle_run
-    v = array_domain_case(form, IT, vector_width=4)
+def test_vector_kernel_close_to_oracle(oracle_long, form, tol):
+    o = oracle_long
+    v = array_domain_case(form, IT_LONG, vector_width=4)
     uref = np.abs(o["u"]).max()
     assert np.abs(v["u"] - o["u"]).max() <= tol * uref
     assert np.abs(v["rho"] - o["rho"]).max() <= tol / 10
     dref = np.abs(o["drag"][..., 0]).max()
+    assert dref > 1e-6
     assert np.abs(v["drag"] - o["drag"]).max() <= tol * dref
+
+@pytest.mark.parametrize("form", ["f16", "bf16"])
+def test_vector_kernel_random_state(form):
+    """From a random (non-smooth) state every node's arithmetic is exercised
+    with distinct values, so any rounding difference between the two kernel
+    shapes shows up after one or two iterations."""
+    for its in (1, 2):
+        print(f"case {form} iterations={its}", flush=True)
+        v, s = one_step_pair(storage=form, iterations=its)
+        assert np.array_equal(v, s), its
