@@ -72,7 +72,7 @@ def inlet_rows(app, n_rows):
     return rows
 
 
-def run_form(app, lat, om_p, om_m, storage, steps, warmup, graph_len, cpt=0):
+def run_form(app, lat, om_p, om_m, storage, steps, warmup, graph_len, cpt=0, vector_width=0):
     import torch
     from lbm_amd import _lib
     from lbm_amd.domain import FusedDomain, pack_inlets
@@ -80,7 +80,8 @@ def run_form(app, lat, om_p, om_m, storage, steps, warmup, graph_len, cpt=0):
     total = warmup + steps
     bc = [_lib.BC_VELOCITY, _lib.BC_VELOCITY, _lib.BC_VELOCITY, _lib.BC_PRESSURE]
     dom = FusedDomain(nx, ny, om_p, om_m, bc, [(o.boundary, o.ibb) for o in app.obstacles],
-                      ibb=True, storage=storage, ring=total + 3, cells_per_thread=cpt)
+                      ibb=True, storage=storage, ring=total + 3, cells_per_thread=cpt,
+                      vector_width=vector_width)
     pool, offs = pack_inlets(inlet_rows(app, total + 2))
     dom.set_inlets(0, offs, pool)
     w = torch.tensor([4. / 9.] + [1. / 9.] * 4 + [1. / 36.] * 4, dtype=torch.float64)
@@ -426,6 +427,8 @@ def main():
     ap.add_argument("--forms", default="f16,f64,f32,bf16")
     ap.add_argument("--graph-len", type=int, default=500)
     ap.add_argument("--cpt", type=int, default=0)
+    ap.add_argument("--vector-width", type=int, default=0,
+                    help="16-bit forms: 4 nodes per work-item (4), one (1) or automatic (0)")
     ap.add_argument("--ny", type=int, default=2048)
     ap.add_argument("--cpu-iterations", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
@@ -447,7 +450,7 @@ def main():
         print(f"# form {f}: {args.warmup} warmup + {args.steps} timed iterations",
               file=sys.stderr, flush=True)
         res[f] = run_form(app, lat, om_p, om_m, f, args.steps, args.warmup, args.graph_len,
-                          args.cpt)
+                          args.cpt, args.vector_width)
         print(f"# form {f}: {res[f]['mlups']:.0f} MLUPS, {res[f]['ms'] / args.steps * 1e3:.1f} "
               f"us/iteration", file=sys.stderr, flush=True)
     ref = res.get("f64")

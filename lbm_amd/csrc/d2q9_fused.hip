@@ -150,6 +150,7 @@ struct Params {
     const int *link_cells; // DRAG: distinct link nodes
     int n_link_cells;
     int tile_shift;        // log2(tile_cells)
+    int rev;               // sweep the XCD slabs last tile first (odd iterations)
 };
 
 enum { MODE_INIT = 0, MODE_STEP = 1, MODE_EPILOGUE = 2, MODE_DRAG = 3 };
@@ -499,11 +500,13 @@ __device__ __forceinline__ void collide_store_h(const Params &p, int cell, const
 // Work-group index -> tile index, so that each XCD (work-groups are dealt
 // round-robin over the 8 XCDs) sweeps one contiguous slab of the lattice and
 // the rows its tiles share stay in that XCD's L2.
-__device__ __forceinline__ int xcd_tile(int b, int nb)
+// With rev set the slab is swept last tile first.
+__device__ __forceinline__ int xcd_tile(int b, int nb, int rev = 0)
 {
     const int q = nb >> 3, rr = nb & 7;
     const int xcd = b & 7, i = b >> 3;
-    return xcd * q + min(xcd, rr) + i;
+    const int len = q + (xcd < rr ? 1 : 0);   // work-groups of this XCD
+    return xcd * q + min(xcd, rr) + (rev ? len - 1 - i : i);
 }
 
 template <int S>
@@ -670,19 +673,23 @@ __global__ __launch_bounds__(256) void k_step(Params p)
     if ((int)blockIdx.x >= p.n_main)
         step_perimeter<S>(p, (int)blockIdx.x - p.n_main, row, macro);
     else
-        step_tile<S, CPT>(p, xcd_tile(blockIdx.x, p.n_main), row, macro);
+        step_tile<S, CPT>(p, xcd_tile(blockIdx.x, p.n_main, p.rev), row, macro);
 }
 
 // 16-bit storage forms, four consecutive nodes per work-item: every plane is
 // read and written with aligned 8-byte accesses.  A plane shifted by one node
 // in y (cy = +-1) takes its missing element from the neighbouring work-item
-// (cross-lane shuffle) or, at the edges of the wave's 256-node segment, from
+// (DPP whole-wave shift) or, at the edges of the wave's 256-node segment, from
 // one extra load.  Needs ny % 4 == 0 and N % (1024*CPT) == 0.
 __device__ __forceinline__ float h16_to_f(uint32_t b, bool bf16)
 {
     if (bf16) return __uint_as_float(b << 16);
     return (float)__builtin_bit_cast(_Float16, (uint16_t)b);
 }
+
+// DPP controls of the whole-wave shifts (GFX9 encoding)
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
 
 __device__ __forceinline__ uint32_t f_to_h16(float v, bool bf16)
 {
@@ -704,21 +711,33 @@ __device__ __forceinline__ void step_tile_v4(const Params &p, int tile, int row,
     for (int c = 0; c < CPT; ++c) {
         const int seg = tile * (1024 * CPT) + c * 1024 + wave * 256;  // wave's first node
         const int cell0 = seg + lane * 4;
-        float h[4][9];
+        // all loads first: nine 8-byte plane loads and, for the six planes
+        // shifted in y, the one element beyond the wave's segment
+        uint2 raw[9];
+        uint32_t edge[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
             const uint16_t *pq = src + ((ptrdiff_t)q * p.N - CX[q] * p.ny);
-            const uint2 a = *reinterpret_cast<const uint2 *>(pq + cell0);
+            raw[q] = *reinterpret_cast<const uint2 *>(pq + cell0);
+            if (CY[q] == 1) edge[q] = pq[seg - 1];
+            else if (CY[q] == -1) edge[q] = pq[seg + 256];
+        }
+        float h[4][9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const uint2 a = raw[q];
             uint32_t e[4] = {a.x & 0xffffu, a.x >> 16, a.y & 0xffffu, a.y >> 16};
             if (CY[q] == 1) {          // source y-1: (prev, e0, e1, e2)
-                uint32_t prev = (uint32_t)__shfl_up((int)e[3], 1, 64);
-                const uint32_t edge = pq[seg - 1];
-                if (lane == 0) prev = edge;
+                // wave_shr:1 -- lane i takes e[3] of lane i-1; lane 0 has no
+                // source lane and keeps the old operand, the edge element
+                const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp(
+                    (int)edge[q], (int)e[3], DPP_WAVE_SHR1, 0xf, 0xf, false);
                 e[3] = e[2]; e[2] = e[1]; e[1] = e[0]; e[0] = prev;
             } else if (CY[q] == -1) {  // source y+1: (e1, e2, e3, next)
-                uint32_t next = (uint32_t)__shfl_down((int)e[0], 1, 64);
-                const uint32_t edge = pq[seg + 256];
-                if (lane == 63) next = edge;
+                // wave_shl:1 -- lane i takes e[0] of lane i+1; lane 63 keeps
+                // the edge element
+                const uint32_t next = (uint32_t)__builtin_amdgcn_update_dpp(
+                    (int)edge[q], (int)e[0], DPP_WAVE_SHL1, 0xf, 0xf, false);
                 e[0] = e[1]; e[1] = e[2]; e[2] = e[3]; e[3] = next;
             }
 #pragma unroll
@@ -778,7 +797,7 @@ __global__ __launch_bounds__(256) void k_step_v4(Params p)
     if ((int)blockIdx.x >= p.n_main)
         step_perimeter<S>(p, (int)blockIdx.x - p.n_main, row, macro);
     else
-        step_tile_v4<S, CPT>(p, xcd_tile(blockIdx.x, p.n_main), row, macro);
+        step_tile_v4<S, CPT>(p, xcd_tile(blockIdx.x, p.n_main, p.rev), row, macro);
 }
 
 // Batched time step: many lattices in one launch.  cases[2*c + parity] holds
@@ -974,6 +993,11 @@ Params base_params(const lbm_fused *h)
     p.tile_shift = h->tile_shift;
     return p;
 }
+
+// Odd iterations sweep the tiles in reverse, so that they start on the nodes
+// the previous iteration wrote last, which may still sit in the Infinity Cache.
+// (8192x2048 obstacle array, FP16 storage: +7% node updates per second.)
+int sweep_rev(int64_t it) { return (int)(it & 1); }
 
 int row_of(const lbm_fused *h, int64_t it)
 {
@@ -1361,6 +1385,7 @@ extern "C" int lbm_step(lbm_fused *h, int64_t it, double *rho, double *u, void *
     p.k = row_of(h, it - 1);
     p.rho_out = rho;
     p.u_out = u;
+    p.rev = sweep_rev(it);
     launch_step(h, p, S(stream));
     LBM_LAUNCH_CHECK();
     return LBM_OK;
@@ -1384,6 +1409,7 @@ extern "C" int lbm_steps(lbm_fused *h, int64_t it0, int n, void *stream)
             p.dst = h->buf[(it0 + k) & 1];
             p.base = h->d_base;
             p.k = k;
+            p.rev = sweep_rev(it0 + k);
             launch_step(h, p, h->cap_stream);
         }
         hipError_t ec = hipStreamEndCapture(h->cap_stream, &graph);
