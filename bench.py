@@ -29,7 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-STORE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2}
+STORE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "f16e": 2}
+# algorithmic bytes per lattice update: nine populations read and written
+# (18 x storage bytes); f16e also reads and writes its half-pair velocity
+BYTES_PER_UPDATE = {f: 18 * b for f, b in STORE_BYTES.items()}
+BYTES_PER_UPDATE["f16e"] = 18 * 2 + 8
 
 
 def build_case(ny=2048, aspect=4, n_obs=64, seed=0):
@@ -419,6 +423,92 @@ def section_adjoint(log):
     return out
 
 
+FORM_OF_STORE = {"0": "f64", "1": "f32", "2": "f16", "3": "bf16", "4": "f16e"}
+
+
+def summarize_profile(pdir, nodes, copy_bytes=2 << 30):
+    """Turn the rocprofv3 output of the three profiling runs under `pdir`
+    (trace/: --kernel-trace --stats, fetch/: --pmc FETCH_SIZE, write/: --pmc
+    WRITE_SIZE, commands.txt: the commands as run) into the text summary and
+    the traffic figures under profiles/.  FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) are scaled by what they read for the copy kernel, which moves
+    `copy_bytes` each way per dispatch."""
+    import csv
+    import glob
+    import re
+
+    def rows(sub, suffix):
+        out = []
+        for f in glob.glob(os.path.join(pdir, sub, "**", "*" + suffix), recursive=True):
+            with open(f) as fh:
+                out += list(csv.DictReader(fh))
+        return out
+
+    def form_of(name):
+        m = re.search(r"k_step(_v4)?<(\d), (\d)>", name)
+        return (FORM_OF_STORE[m.group(2)], m.group(1) is not None, int(m.group(3))) if m else None
+
+    stats = {r["Name"]: r for r in rows("trace", "kernel_stats.csv")}
+    counters = {}
+    for sub, cname in (("fetch", "FETCH_SIZE"), ("write", "WRITE_SIZE")):
+        acc = {}
+        for r in rows(sub, "counter_collection.csv"):
+            if r["Counter_Name"] == cname:
+                acc.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        counters[cname] = {k: sum(v) / len(v) for k, v in acc.items()}
+    copy = [k for k in counters["FETCH_SIZE"] if "k_copy16" in k]
+    assert copy, "no copy-kernel dispatch in the FETCH_SIZE run"
+    fetch_factor = copy_bytes / 1024.0 / counters["FETCH_SIZE"][copy[0]]
+    write_factor = copy_bytes / 1024.0 / counters["WRITE_SIZE"][copy[0]]
+    with open(os.path.join(pdir, "commands.txt")) as fh:
+        commands = fh.read().strip()
+    lines = ["# Fused D2Q9 step kernels on the 8192x2048 obstacle array (64 IBB squares)",
+             "", "Commands (repository root; rocprofv3 run from /tmp with TMPDIR=/tmp):", "",
+             "```", commands, "```", "",
+             f"Counter calibration on the copy kernel (k_copy16, {copy_bytes >> 20} MiB read and "
+             f"{copy_bytes >> 20} MiB written per dispatch): FETCH_SIZE x {fetch_factor:.3f}, "
+             f"WRITE_SIZE x {write_factor:.3f}.", "",
+             "| kernel | form | calls | avg us | min us | max us | algorithmic MB/launch "
+             "| HBM read MB | HBM write MB | traffic / algorithmic | algorithmic GB/s |",
+             "|---|---|---|---|---|---|---|---|---|---|---|"]
+    traffic = {}
+    for name, st in sorted(stats.items()):
+        fm = form_of(name)
+        if fm is None:
+            continue
+        form, v4, cpt = fm
+        alg = float(BYTES_PER_UPDATE[form]) * nodes
+        avg_us = float(st["AverageNs"]) / 1e3
+        fb = counters["FETCH_SIZE"].get(name)
+        wb = counters["WRITE_SIZE"].get(name)
+        rd = fb * 1024.0 * fetch_factor if fb is not None else None
+        wr = wb * 1024.0 * write_factor if wb is not None else None
+        tot = rd + wr if rd is not None and wr is not None else None
+        lines.append(
+            f"| `{name}` | {form}{' 4-node' if v4 else ''} cpt {cpt} | {st['Calls']} | "
+            f"{avg_us:.2f} | {float(st['MinNs']) / 1e3:.2f} | {float(st['MaxNs']) / 1e3:.2f} | "
+            f"{alg / 1e6:.1f} | {rd / 1e6 if rd else float('nan'):.1f} | "
+            f"{wr / 1e6 if wr else float('nan'):.1f} | "
+            f"{tot / alg if tot else float('nan'):.3f} | {alg / (avg_us * 1e-6) / 1e9:.1f} |")
+        traffic[form] = {"kernel": name, "avg_duration_us": round(avg_us, 3),
+                         "algorithmic_bytes_per_launch": alg,
+                         "hbm_read_bytes_per_launch": rd, "hbm_write_bytes_per_launch": wr,
+                         "hbm_bytes_per_launch": tot}
+    lines += ["", "Other kernels (kernel trace):", "",
+              "| kernel | calls | avg us |", "|---|---|---|"]
+    for name, st in sorted(stats.items()):
+        if form_of(name) is None:
+            lines.append(f"| `{name[:90]}` | {st['Calls']} | {float(st['AverageNs']) / 1e3:.2f} |")
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    with open(os.path.join(ROOT, "profiles", "k_step_obstacle_array_8192x2048.md"), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    out = dict(traffic.get("f16", {}), fetch_factor=fetch_factor, write_factor=write_factor,
+               forms=traffic)
+    with open(os.path.join(ROOT, "profiles", "fused_f16_obstacle_array_traffic.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -436,7 +526,12 @@ def main():
                     help="c3 obstacle array (headline), c1 cavity, c2 turek, c4 batch, c5 adjoint")
     ap.add_argument("--turek-iterations", type=int, default=200000)
     ap.add_argument("--batch-iterations", type=int, default=5000)
+    ap.add_argument("--summarize-profile", metavar="DIR",
+                    help="write profiles/ summaries from rocprofv3 output under DIR and exit")
     args = ap.parse_args()
+    if args.summarize_profile:
+        summarize_profile(args.summarize_profile, nodes=8192 * args.ny)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the domain decomposition is described in DESIGN.md)", file=sys.stderr)
@@ -457,11 +552,11 @@ def main():
     # drag error: time-averaged drag of each obstacle over the last 1000 iterations
     forms_out = {}
     for f, r in res.items():
-        sb = STORE_BYTES[f]
-        gbs = r["mlups"] * 1e6 * 18 * sb / 1e9
+        bpu = BYTES_PER_UPDATE[f]
+        gbs = r["mlups"] * 1e6 * bpu / 1e9
         entry = {"mlups": round(r["mlups"], 1), "us_per_iteration": round(r["ms"] / args.steps * 1e3, 2),
                  "algorithmic_GBps": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-                 "bytes_per_update": 18 * sb, "mass": r["mass"]}
+                 "bytes_per_update": bpu, "mass": r["mass"]}
         if ref is not None and f != "f64":
             # time-averaged momentum-exchange force of each obstacle over the last
             # iterations; errors relative to the largest FP64 drag of the array
@@ -479,8 +574,7 @@ def main():
         forms_out[f] = entry
     head = "f16" if "f16" in res else forms[0]
     h = res[head]
-    sb = STORE_BYTES[head]
-    achieved = h["mlups"] * 1e6 * 18 * sb / 1e9
+    achieved = h["mlups"] * 1e6 * BYTES_PER_UPDATE[head] / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "fused_f16_obstacle_array_traffic.json")
     if os.path.exists(tpath):

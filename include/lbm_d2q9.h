@@ -41,6 +41,10 @@ extern "C" {
 #define LBM_STORE_F32   1   /* float storage of f_q - w_q, float arithmetic */
 #define LBM_STORE_F16   2   /* IEEE half storage of f_q - w_q, float arithmetic */
 #define LBM_STORE_BF16  3   /* bfloat16 storage of f_q - w_q, float arithmetic */
+#define LBM_STORE_F16E  4   /* IEEE half storage of f_q - w_q - E_q(v), E_q the
+                               equilibrium deviation at the node's velocity v,
+                               which a tenth plane keeps as a half pair; float
+                               arithmetic (fused and batched paths) */
 
 /* Wall kinds of the fused boundary-condition plan. */
 #define LBM_BC_NONE      0
@@ -154,7 +158,9 @@ typedef struct {
     int32_t ring;               /* rows of the inlet table and of the drag ring */
     int32_t cells_per_thread;   /* 1, 2, 4 or 8 (0 = library default) */
     int32_t vector_width;       /* 16-bit forms: 4 = four consecutive nodes per
-                                   work-item (default when ny % 4 == 0), 1 = one */
+                                   work-item with 8-byte accesses (taken when
+                                   ny % 4 == 0 and nx*ny % (1024*cpt) == 0),
+                                   0 or 1 = one node per work-item */
 } lbm_fused_desc;
 
 int  lbm_fused_create(const lbm_fused_desc *desc, lbm_fused **out);

@@ -40,45 +40,70 @@ int set_error(int code, const std::string &msg)
 }
 
 // ---------------------------------------------------------------- storage --
-// ld/st move full populations f_q (the reference's variables).  The float
-// forms store the deviation h_q = f_q - w_q (IEEE half, bfloat16 or float);
-// ldh/sth move that deviation without conversion to f.
+// A state buffer holds nine population planes of N nodes (plane q, node
+// x*ny + y).  ld(p, N, q, node) returns the population f_q; the float forms
+// store a deviation and ldh returns h_q = f_q - w_q.
+//   F64             f_q in double
+//   F32, F16, BF16  h_q in float, IEEE half or bfloat16
+//   F16E            g_q = h_q - E_q(v) in IEEE half, with
+//                   E_q(v) = w_q (3 c_q.v + 4.5 (c_q.v)^2 - 1.5 v.v)
+//                   the equilibrium deviation at the node's velocity v rounded
+//                   to half precision; a tenth plane holds v as packed (vx, vy)
+//                   half pairs.  g_q is the density and non-equilibrium part
+//                   of h_q (~1e-3 against ~2e-2 for h_q in a u ~ 0.05 flow), so
+//                   half precision resolves it 10-30 times finer.  Writer and
+//                   reader evaluate E_q from the same half-precision v with the
+//                   same operations, so the shift cancels exactly.
 template <int S> struct Store;
+
+// Float -> half conversions take their operand through an empty asm, so the
+// compiler cannot fold the preceding multiply-add into a mixed-precision FMA
+// that rounds to half once (v_fma_mixlo_f16): every kernel then rounds twice,
+// to float and to half, and all kernel shapes store identical values.
+__device__ __forceinline__ _Float16 to_half(float v)
+{
+    asm("" : "+v"(v));
+    return (_Float16)v;
+}
+
+// F16E: offset (in halves) of the velocity plane, 256-byte aligned
+__host__ __device__ constexpr size_t vel_offset(int N) { return ((size_t)9 * N + 127) / 128 * 128; }
 
 template <> struct Store<LBM_STORE_F64> {
     using s_t = double;
     using T = double;
-    static __device__ __forceinline__ T ld(const s_t *p, size_t i, int) { return p[i]; }
+    static __device__ __forceinline__ T ld(const s_t *p, int N, int q, size_t c)
+    {
+        return p[(size_t)q * N + c];
+    }
     static __device__ __forceinline__ void st(s_t *p, size_t i, int, T v) { p[i] = v; }
 };
 
 template <> struct Store<LBM_STORE_F32> {
     using s_t = float;
     using T = float;
-    static __device__ __forceinline__ float ldh(const s_t *p, size_t i) { return p[i]; }
-    static __device__ __forceinline__ void sth(s_t *p, size_t i, float h) { p[i] = h; }
-    static __device__ __forceinline__ T ld(const s_t *p, size_t i, int q)
+    static __device__ __forceinline__ float ldh(const s_t *p, int N, int q, size_t c)
     {
-        return weight<float>(q) + p[i];
+        return p[(size_t)q * N + c];
     }
-    static __device__ __forceinline__ void st(s_t *p, size_t i, int q, T v)
+    static __device__ __forceinline__ void sth(s_t *p, size_t i, float h) { p[i] = h; }
+    static __device__ __forceinline__ T ld(const s_t *p, int N, int q, size_t c)
     {
-        p[i] = v - weight<float>(q);
+        return weight<float>(q) + ldh(p, N, q, c);
     }
 };
 
 template <> struct Store<LBM_STORE_F16> {
     using s_t = _Float16;
     using T = float;
-    static __device__ __forceinline__ float ldh(const s_t *p, size_t i) { return (float)p[i]; }
-    static __device__ __forceinline__ void sth(s_t *p, size_t i, float h) { p[i] = (_Float16)h; }
-    static __device__ __forceinline__ T ld(const s_t *p, size_t i, int q)
+    static __device__ __forceinline__ float ldh(const s_t *p, int N, int q, size_t c)
     {
-        return weight<float>(q) + (float)p[i];
+        return (float)p[(size_t)q * N + c];
     }
-    static __device__ __forceinline__ void st(s_t *p, size_t i, int q, T v)
+    static __device__ __forceinline__ void sth(s_t *p, size_t i, float h) { p[i] = to_half(h); }
+    static __device__ __forceinline__ T ld(const s_t *p, int N, int q, size_t c)
     {
-        p[i] = (_Float16)(v - weight<float>(q));
+        return weight<float>(q) + ldh(p, N, q, c);
     }
 };
 
@@ -92,18 +117,48 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float v)
 template <> struct Store<LBM_STORE_BF16> {
     using s_t = uint16_t;
     using T = float;
-    static __device__ __forceinline__ float ldh(const s_t *p, size_t i)
+    static __device__ __forceinline__ float ldh(const s_t *p, int N, int q, size_t c)
     {
-        return __uint_as_float((uint32_t)p[i] << 16);
+        return __uint_as_float((uint32_t)p[(size_t)q * N + c] << 16);
     }
     static __device__ __forceinline__ void sth(s_t *p, size_t i, float h) { p[i] = f32_to_bf16(h); }
-    static __device__ __forceinline__ T ld(const s_t *p, size_t i, int q)
+    static __device__ __forceinline__ T ld(const s_t *p, int N, int q, size_t c)
     {
-        return weight<float>(q) + ldh(p, i);
+        return weight<float>(q) + ldh(p, N, q, c);
     }
-    static __device__ __forceinline__ void st(s_t *p, size_t i, int q, T v)
+};
+
+// F16E: E_q(v), explicit fmaf only, so every caller rounds identically
+__device__ __forceinline__ float eq_shift(int q, float vx, float vy)
+{
+    const float vv = __builtin_fmaf(vy, vy, vx * vx);
+    const float cu = (float)CX[q] * vx + (float)CY[q] * vy;
+    const float t = __builtin_fmaf(4.5f * cu, cu, 3.0f * cu);
+    return weight<float>(q) * __builtin_fmaf(-1.5f, vv, t);
+}
+
+__device__ __forceinline__ void vel_unpack(uint32_t v, float &vx, float &vy)
+{
+    vx = (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu));
+    vy = (float)__builtin_bit_cast(_Float16, (uint16_t)(v >> 16));
+}
+
+template <> struct Store<LBM_STORE_F16E> {
+    using s_t = _Float16;
+    using T = float;
+    static __device__ __forceinline__ const uint32_t *vel(const s_t *p, int N)
     {
-        p[i] = f32_to_bf16(v - weight<float>(q));
+        return reinterpret_cast<const uint32_t *>(p + vel_offset(N));
+    }
+    static __device__ __forceinline__ float ldh(const s_t *p, int N, int q, size_t c)
+    {
+        float vx, vy;
+        vel_unpack(vel(p, N)[c], vx, vy);
+        return eq_shift(q, vx, vy) + (float)p[(size_t)q * N + c];
+    }
+    static __device__ __forceinline__ T ld(const s_t *p, int N, int q, size_t c)
+    {
+        return weight<float>(q) + ldh(p, N, q, c);
     }
 };
 
@@ -191,7 +246,7 @@ __device__ __forceinline__ void pull_checked(const Params &p, int x, int y,
     for (int q = 0; q < 9; ++q) {
         const int sx = x - CX[q], sy = y - CY[q];
         f[q] = (sx >= 0 && sx < p.nx && sy >= 0 && sy < p.ny)
-                   ? St::ld(src, (size_t)q * p.N + (size_t)sx * p.ny + sy, q)
+                   ? St::ld(src, p.N, q, (size_t)sx * p.ny + sy)
                    : T(0);
     }
 }
@@ -206,10 +261,10 @@ __device__ __forceinline__ float link_h(const Params &p, int cell, const LinkRec
 {
     using St = Store<S>;
     const auto *src = static_cast<const typename St::s_t *>(p.src);
-    const float A = St::ldh(src, (size_t)L.q * p.N + cell);
+    const float A = St::ldh(src, p.N, L.q, cell);
     if (!L.mode) return A;
-    const float B = St::ldh(src, (size_t)L.planeB * p.N + L.cellB);
-    const float C = St::ldh(src, (size_t)L.planeC * p.N + L.cellC);
+    const float B = St::ldh(src, p.N, L.planeB, L.cellB);
+    const float C = St::ldh(src, p.N, L.planeC, L.cellC);
     return ((float)L.c0 * A + (float)L.c1 * B) + (float)L.c2 * C;
 }
 
@@ -226,11 +281,11 @@ __device__ __forceinline__ void apply_links(const Params &p, int cell, int lo, i
         const LinkRec L = p.links[k];
         T v;
         if constexpr (S == LBM_STORE_F64) {
-            const T A = St::ld(src, (size_t)L.q * p.N + cell, L.q);
+            const T A = St::ld(src, p.N, L.q, cell);
             v = A;
             if (L.mode) {
-                const T B = St::ld(src, (size_t)L.planeB * p.N + L.cellB, L.planeB);
-                const T C = St::ld(src, (size_t)L.planeC * p.N + L.cellC, L.planeC);
+                const T B = St::ld(src, p.N, L.planeB, L.cellB);
+                const T C = St::ld(src, p.N, L.planeC, L.cellC);
                 v = (T(L.c0) * A + T(L.c1) * B) + T(L.c2) * C;
             }
         } else {
@@ -347,14 +402,14 @@ __device__ __forceinline__ void store_drag_range(const Params &p, int cell, int 
 #pragma unroll
             for (int q = 1; q < 9; ++q)
                 if (q == L.qb) fb = f[q];
-            v = St::ld(src, (size_t)L.q * p.N + cell, L.q) + fb;
+            v = St::ld(src, p.N, L.q, cell) + fb;
         } else {
             // f[qb] = fl(w + h) with |h| < w/2, so f[qb] - w is exact (Sterbenz)
             float hb = 0.0f;
 #pragma unroll
             for (int q = 1; q < 9; ++q)
                 if (q == L.qb) hb = f[q] - weight<float>(q);
-            const double hA = (double)St::ldh(src, (size_t)L.q * p.N + cell);
+            const double hA = (double)St::ldh(src, p.N, L.q, cell);
             v = 2.0 * weight<double>(L.q) + (hA + (double)hb);
         }
         p.drag_ring[(size_t)row * p.n_links + L.dslot] = v;
@@ -374,7 +429,7 @@ __device__ __forceinline__ void store_drag_range_h(const Params &p, int cell, in
 #pragma unroll
         for (int q = 1; q < 9; ++q)
             if (q == L.qb) hb = h[q];
-        const double hA = (double)St::ldh(src, (size_t)L.q * p.N + cell);
+        const double hA = (double)St::ldh(src, p.N, L.q, cell);
         p.drag_ring[(size_t)row * p.n_links + L.dslot] =
             2.0 * weight<double>(L.q) + (hA + (double)hb);
     }
@@ -493,8 +548,21 @@ __device__ __forceinline__ void collide_store_h(const Params &p, int cell, const
         p.u_out[cell] = (double)ux;
         p.u_out[(size_t)p.N + cell] = (double)uy;
     }
+    if constexpr (S == LBM_STORE_F16E) {
+        // the node's velocity in half precision, then g_q = h_q - E_q(v)
+        const _Float16 hx = to_half(ux), hy = to_half(uy);
+        const uint32_t v = (uint32_t)__builtin_bit_cast(uint16_t, hx) |
+                           ((uint32_t)__builtin_bit_cast(uint16_t, hy) << 16);
+        reinterpret_cast<uint32_t *>(dst + vel_offset(p.N))[cell] = v;
+        float vx, vy;
+        vel_unpack(v, vx, vy);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) St::sth(dst, (size_t)q * p.N + cell, h[q]);
+        for (int q = 0; q < 9; ++q)
+            dst[(size_t)q * p.N + cell] = to_half(h[q] - eq_shift(q, vx, vy));
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) St::sth(dst, (size_t)q * p.N + cell, h[q]);
+    }
 }
 
 // Work-group index -> tile index, so that each XCD (work-groups are dealt
@@ -560,7 +628,7 @@ __global__ __launch_bounds__(256) void k_fused(Params p)
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
                 p.g_out[(size_t)q * p.N + cell] = (double)f[q];
-                p.gup_out[(size_t)q * p.N + cell] = (double)St::ld(src, (size_t)q * p.N + cell, q);
+                p.gup_out[(size_t)q * p.N + cell] = (double)St::ld(src, p.N, q, cell);
             }
             if (!interior && p.rho_out) {
                 p.rho_out[cell] = (double)r;
@@ -627,11 +695,11 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
         const int lc = in[c] ? cell : p.ny + 1;
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
-            const s_t *pq = src + ((ptrdiff_t)q * p.N - (CX[q] * p.ny + CY[q]));
+            const size_t sc = (size_t)(lc - (CX[q] * p.ny + CY[q]));   // source node
             if constexpr (S == LBM_STORE_F64)
-                v[c][q] = pq[lc];
+                v[c][q] = St::ld(src, p.N, q, sc);
             else
-                v[c][q] = St::ldh(pq, lc);
+                v[c][q] = St::ldh(src, p.N, q, sc);
         }
     }
 #pragma unroll
@@ -694,7 +762,7 @@ constexpr int DPP_WAVE_SHR1 = 0x138;
 __device__ __forceinline__ uint32_t f_to_h16(float v, bool bf16)
 {
     if (bf16) return f32_to_bf16(v);
-    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v);
+    return (uint32_t)__builtin_bit_cast(uint16_t, to_half(v));
 }
 
 template <int S, int CPT>
@@ -871,7 +939,7 @@ __global__ __launch_bounds__(256) void k_mass_partial(const void *buf, int N, do
     for (int c = blockIdx.x * 256 + threadIdx.x; c < N; c += gridDim.x * 256) {
         double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < 9; ++q) s += (double)St::ld(src, (size_t)q * N + c, q);
+        for (int q = 0; q < 9; ++q) s += (double)St::ld(src, N, q, c);
         acc += s;
     }
     sh[threadIdx.x] = acc;
@@ -898,6 +966,13 @@ __global__ void k_set_int(int *p, int v)
 
 // ------------------------------------------------------------------ host --
 size_t store_bytes(int s) { return s == LBM_STORE_F64 ? 8 : (s == LBM_STORE_F32 ? 4 : 2); }
+
+// bytes of one state buffer (nine planes; F16E adds its velocity plane)
+size_t state_bytes(int s, int N)
+{
+    if (s == LBM_STORE_F16E) return 2 * vel_offset(N) + 4 * (size_t)N;
+    return store_bytes(s) * 9 * (size_t)N;
+}
 
 struct GraphKey {
     int n, parity;
@@ -1044,7 +1119,8 @@ void launch_step(const lbm_fused *h, const Params &p, hipStream_t s)
     case LBM_STORE_F64: launch_step_s<LBM_STORE_F64>(h, p, s); break;
     case LBM_STORE_F32: launch_step_s<LBM_STORE_F32>(h, p, s); break;
     case LBM_STORE_F16: launch_step_s<LBM_STORE_F16>(h, p, s); break;
-    default: launch_step_s<LBM_STORE_BF16>(h, p, s); break;
+    case LBM_STORE_BF16: launch_step_s<LBM_STORE_BF16>(h, p, s); break;
+    default: launch_step_s<LBM_STORE_F16E>(h, p, s); break;
     }
 }
 
@@ -1054,7 +1130,8 @@ template <int MODE> void launch_fused(const lbm_fused *h, const Params &p, hipSt
     case LBM_STORE_F64: launch_fused_s<LBM_STORE_F64, MODE>(h, p, s); break;
     case LBM_STORE_F32: launch_fused_s<LBM_STORE_F32, MODE>(h, p, s); break;
     case LBM_STORE_F16: launch_fused_s<LBM_STORE_F16, MODE>(h, p, s); break;
-    default: launch_fused_s<LBM_STORE_BF16, MODE>(h, p, s); break;
+    case LBM_STORE_BF16: launch_fused_s<LBM_STORE_BF16, MODE>(h, p, s); break;
+    default: launch_fused_s<LBM_STORE_F16E, MODE>(h, p, s); break;
     }
 }
 
@@ -1066,7 +1143,8 @@ void launch_drag_only(const lbm_fused *h, const Params &p, hipStream_t s)
     case LBM_STORE_F64: k_drag_only<LBM_STORE_F64><<<grid, 256, 0, s>>>(p); break;
     case LBM_STORE_F32: k_drag_only<LBM_STORE_F32><<<grid, 256, 0, s>>>(p); break;
     case LBM_STORE_F16: k_drag_only<LBM_STORE_F16><<<grid, 256, 0, s>>>(p); break;
-    default: k_drag_only<LBM_STORE_BF16><<<grid, 256, 0, s>>>(p); break;
+    case LBM_STORE_BF16: k_drag_only<LBM_STORE_BF16><<<grid, 256, 0, s>>>(p); break;
+    default: k_drag_only<LBM_STORE_F16E><<<grid, 256, 0, s>>>(p); break;
     }
 }
 
@@ -1234,7 +1312,7 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     if (d->nx < 4 || d->ny < 4) return set_error(LBM_ERR_ARG, "nx and ny must be >= 4");
     if ((long long)d->nx * d->ny * 9 >= (1LL << 31))
         return set_error(LBM_ERR_ARG, "lattice too large for 32-bit population indices");
-    if (d->storage < 0 || d->storage > 3) return set_error(LBM_ERR_ARG, "bad storage");
+    if (d->storage < 0 || d->storage > LBM_STORE_F16E) return set_error(LBM_ERR_ARG, "bad storage");
     if (d->ring < 2) return set_error(LBM_ERR_ARG, "ring must be >= 2");
     if (d->n_links < 0 || (d->n_links > 0 && (!d->links_host || !d->ibb_host)))
         return set_error(LBM_ERR_ARG, "bad link arrays");
@@ -1254,12 +1332,16 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     h->N = d->nx * d->ny;
     h->storage = d->storage;
     h->cpt = d->cells_per_thread;
-    if (h->cpt == 0) h->cpt = (d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16) ? 4 : 2;
+    if (h->cpt == 0)
+        h->cpt = (d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16 ||
+                  d->storage == LBM_STORE_F16E) ? 4 : 2;
     if (h->cpt != 1 && h->cpt != 2 && h->cpt != 4 && h->cpt != 8)
         return set_error(LBM_ERR_ARG, "cells_per_thread must be 1, 2, 4 or 8");
-    // 16-bit forms use four nodes per work-item when the lattice allows it
+    // 16-bit forms: four nodes per work-item on request, when the lattice
+    // allows it (one node per work-item with four per tile row is faster on
+    // the 8192x2048 obstacle array: 122 against 111 GLUPS)
     const bool sixteen = d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16;
-    if (sixteen && d->ny % 4 == 0 && d->vector_width != 1) {
+    if (sixteen && d->ny % 4 == 0 && d->vector_width == 4) {
         const int cpt = (h->cpt >= 4) ? 4 : h->cpt;
         if ((long long)d->nx * d->ny % (1024LL * cpt) == 0) {
             h->vec4 = true;
@@ -1287,7 +1369,7 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     h->n_obs = d->n_obs;
     h->ring = d->ring;
     int e;
-    const size_t sb = store_bytes(d->storage) * 9 * (size_t)h->N;
+    const size_t sb = state_bytes(d->storage, h->N);
     if ((e = dmalloc(h.get(), &h->buf[0], sb)) || (e = dmalloc(h.get(), &h->buf[1], sb))) {
         free_all(h.get());
         return e;
@@ -1489,7 +1571,7 @@ extern "C" int lbm_drag_read(lbm_fused *h, int64_t it0, int n, double *out, void
 extern "C" int lbm_fused_save(lbm_fused *h, int64_t it, void *stream)
 {
     if (!h || it < 0) return set_error(LBM_ERR_ARG, "lbm_fused_save: bad argument");
-    const size_t sb = store_bytes(h->storage) * 9 * (size_t)h->N;
+    const size_t sb = state_bytes(h->storage, h->N);
     if (!h->ckpt)
         if (int e = dmalloc(h, &h->ckpt, sb)) return e;
     LBM_HIP_TRY(hipMemcpyAsync(h->ckpt, h->buf[it & 1], sb, hipMemcpyDeviceToDevice, S(stream)));
@@ -1501,7 +1583,7 @@ extern "C" int lbm_fused_restore(lbm_fused *h, int64_t it, void *stream)
 {
     if (!h || it < 0 || it != h->ckpt_it || !h->ckpt)
         return set_error(LBM_ERR_ARG, "lbm_fused_restore: no checkpoint of that iteration");
-    const size_t sb = store_bytes(h->storage) * 9 * (size_t)h->N;
+    const size_t sb = state_bytes(h->storage, h->N);
     LBM_HIP_TRY(hipMemcpyAsync(h->buf[it & 1], h->ckpt, sb, hipMemcpyDeviceToDevice, S(stream)));
     return LBM_OK;
 }
@@ -1519,7 +1601,8 @@ extern "C" int lbm_mass(lbm_fused *h, int64_t it, double *out, void *stream)
     case LBM_STORE_F64: k_mass_partial<LBM_STORE_F64><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
     case LBM_STORE_F32: k_mass_partial<LBM_STORE_F32><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
     case LBM_STORE_F16: k_mass_partial<LBM_STORE_F16><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
-    default: k_mass_partial<LBM_STORE_BF16><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
+    case LBM_STORE_BF16: k_mass_partial<LBM_STORE_BF16><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
+    default: k_mass_partial<LBM_STORE_F16E><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
     }
     LBM_LAUNCH_CHECK();
     k_sum_serial<<<1, 64, 0, S(stream)>>>(h->d_part, nb, h->d_part + nb);
@@ -1592,7 +1675,8 @@ void launch_batch(const lbm_batch *b, int k, int parity, hipStream_t s)
     case LBM_STORE_F64: launch_batch_t<LBM_STORE_F64>(b, k, parity, s); break;
     case LBM_STORE_F32: launch_batch_t<LBM_STORE_F32>(b, k, parity, s); break;
     case LBM_STORE_F16: launch_batch_t<LBM_STORE_F16>(b, k, parity, s); break;
-    default: launch_batch_t<LBM_STORE_BF16>(b, k, parity, s); break;
+    case LBM_STORE_BF16: launch_batch_t<LBM_STORE_BF16>(b, k, parity, s); break;
+    default: launch_batch_t<LBM_STORE_F16E>(b, k, parity, s); break;
     }
 }
 
@@ -1608,8 +1692,8 @@ extern "C" int lbm_batch_create(const lbm_batch_desc *d, lbm_batch **out)
     b->storage = d->storage;
     const int n = d->n_cases;
     // one vector width for every case: four nodes per work-item only if all allow it
-    int vw = d->vector_width;
-    if (vw != 1 && (d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16)) {
+    int vw = d->vector_width == 4 ? 4 : 1;
+    if (vw == 4 && (d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16)) {
         const int cpt = d->cells_per_thread >= 4 || d->cells_per_thread == 0 ? 4 : d->cells_per_thread;
         for (int c = 0; c < n; ++c) {
             const lbm_case_desc &cd = d->cases[c];

@@ -15,7 +15,7 @@ without liblbm_d2q9.so these methods raise.  ``lbm_amd.run.run`` drives the
 fused time step instead of these per-call kernels.
 
 Extension: an app attribute ``precision`` ('f64' default, 'f32', 'f16',
-'bf16') selects the population storage of the fused time step.
+'bf16', 'f16e') selects the population storage of the fused time step.
 """
 import math
 import os

@@ -66,13 +66,14 @@ def _final(dom, nx, ny, last):
                 g_up=gup.cpu().numpy())
 
 
-def _single(cs, amp_c, storage):
+def _single(cs, amp_c, storage, vector_width=0, cpt=0):
     """One case alone through FusedDomain with a pool inlet table of the same values."""
     nx, ny = cs["nx"], cs["ny"]
     links = cs.get("links")
     obs = [(links, cs["ibb"])] if links is not None else []
     dom = FusedDomain(nx, ny, cs["om_p"], cs["om_m"], cs["bc"], obs, ibb=links is not None,
-                      storage=storage, ring=IT + 2)
+                      storage=storage, ring=IT + 2, vector_width=vector_width,
+                      cells_per_thread=cpt)
     rows = []
     profs = cs["profiles"]
     for k in range(IT + 1):
@@ -109,6 +110,38 @@ def test_batch_fp64_equals_single_runs(kw):
     outs = _batch(cases, amp, "f64", **kw)
     for c, cs in enumerate(cases):
         ref = _single(cs, amp[:, c], "f64")
+        for k in ("rho", "u", "g", "g_up"):
+            assert np.array_equal(outs[c][k], ref[k]), (c, k)
+
+
+@pytest.mark.parametrize("storage,vw", [("f16", 1), ("f16", 4), ("bf16", 1), ("bf16", 4),
+                                        ("f16e", 1)])
+def test_batch_16bit_equals_single_runs(storage, vw):
+    """16-bit storage, one or four nodes per work-item: the four-node kernel
+    needs ny % 4 == 0 and nx*ny a multiple of 1024 (one group of four per
+    work-item), which these cases meet."""
+    cases = cavity_ensemble(n_cases=3, seed=5, n_lo=32, n_hi=32)
+    cases[1] = cavity_ensemble(n_cases=1, seed=6, n_lo=64, n_hi=64)[0]
+    for c in cases:
+        c["sigma"] = 15
+    ch = _cases()[-1]
+    ny = 32
+    ch.update(ny=ny, nx=64, links=ch["links"][ch["links"][:, 1] < ny - 1])
+    ch["ibb"] = np.full(len(ch["links"]), 0.3)
+    ch["ibb_on"] = np.ones(len(ch["links"]), np.uint8)
+    ch["obs_offsets"] = [0, len(ch["links"])]
+    yy = np.arange(ny)
+    left = np.zeros((2, ny))
+    left[0] = 4.0 * yy * (ny - 1 - yy) / (ny - 1) ** 2
+    ch["profiles"] = [None, left, None, None, np.ones(ny)]
+    cases.append(ch)
+    amp = _amp(cases, IT + 1)
+    b = Batch(cases, storage=storage, ring=IT + 2, vector_width=vw, cells_per_thread=1)
+    assert b.vector_width == vw
+    b.close()
+    outs = _batch(cases, amp, storage, vector_width=vw, cells_per_thread=1)
+    for c, cs in enumerate(cases):
+        ref = _single(cs, amp[:, c], storage, vector_width=vw, cpt=1)
         for k in ("rho", "u", "g", "g_up"):
             assert np.array_equal(outs[c][k], ref[k]), (c, k)
 

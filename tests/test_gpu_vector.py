@@ -48,8 +48,12 @@ def oracle_long():
     return array_oracle_case(ref, IT_LONG)
 
 
-@pytest.mark.parametrize("form,tol", [("f16", 2e-2), ("bf16", 1e-1)])
+@pytest.mark.parametrize("form,tol", [("f16", 2e-2), ("bf16", 1e-1), ("f16e", 2e-2)])
 def test_vector_kernel_close_to_oracle(oracle_long, form, tol):
+    """(f16e has no four-node kernel and runs the one-node kernel.  This
+    impulsively started flow is dominated by pressure waves, whose density
+    part the velocity shift of f16e does not remove, so f16e gets the f16
+    tolerance here; its gain shows on steady and periodic flows.)"""
     o = oracle_long
     v = array_domain_case(form, IT_LONG, vector_width=4)
     uref = np.abs(o["u"]).max()
