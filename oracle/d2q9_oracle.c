@@ -21,6 +21,21 @@
  */
 #include <stdint.h>
 #include <stddef.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads the node loops run on (OpenMP over x; every iteration writes its
+ * own nodes, so results do not depend on the thread count).  Sequential
+ * sums (drag) stay on one thread. */
+int oracle_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 static const int CX[9] = {0, 1, -1, 0, 0, 1, -1, -1, 1};   /* lattice.py:135-139 */
 static const int CY[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
@@ -52,6 +67,7 @@ static int wrap(long long k, long long n, long long *out)
  * q order is the reference's value whatever the BLAS does with FMA. */
 void oracle_macro(int nx, int ny, const double *g, double *rho, double *u)
 {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j) {
             double r = G(g, 0, i, j);
@@ -71,6 +87,7 @@ void oracle_macro(int nx, int ny, const double *g, double *rho, double *u)
 void oracle_equilibrium(int nx, int ny, const double *u, const double *rho,
                         double *g_eq)
 {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j) {
             double ux = U(u, 0, i, j), uy = U(u, 1, i, j);
@@ -94,12 +111,14 @@ void oracle_col_str(int nx, int ny, double *g, const double *g_eq, double *g_up,
     const double a2 = 0.5 * (om_p - om_m);
     const double a3 = 0.5 * (om_p + om_m);
     const double a4 = 0.5 * (om_p - om_m);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j) {
             /* q = 0, nb.py:25-26 */
             double v0 = (1.0 - om_p) * G(g, 0, i, j) + om_p * G(g_eq, 0, i, j);
             G(g_up, 0, i, j) = v0;
         }
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int q = 1; q < 9; ++q) {
@@ -108,11 +127,13 @@ void oracle_col_str(int nx, int ny, double *g, const double *g_eq, double *g_up,
                 G(g_up, q, i, j) = a1 * G(g, q, i, j) - a2 * G(g, qb, i, j)
                                  + a3 * G(g_eq, q, i, j) + a4 * G(g_eq, qb, i, j);
             }
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j) G(g, 0, i, j) = G(g_up, 0, i, j);
     /* nb.py:38-45 */
     for (int q = 1; q < 9; ++q) {
         int cx = CX[q], cy = CY[q];
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < nx; ++i) {
             int si = i - cx;
             if (si < 0 || si > lx) continue;

@@ -71,8 +71,15 @@ def load():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = i if name in ("oracle_drag_sums", "oracle_bounce_back_obstacle") else None
+    lib.oracle_threads.argtypes = []
+    lib.oracle_threads.restype = i
     _lib = lib
     return lib
+
+
+def threads():
+    """Threads of the oracle's node loops (OpenMP, OMP_NUM_THREADS)."""
+    return int(load().oracle_threads())
 
 
 def _p(a):
