@@ -260,13 +260,16 @@ __global__ __launch_bounds__(256) void k_pbb_fwd(PbbParams p)
 }
 
 // ------------------------------------------------------------- adjoint --
-// Transposed boundary step of perimeter node (x, y): hbar (adjoint of its
+// Transposed boundary step of a perimeter node: hbar (adjoint of its
 // post-boundary deviations) -> adjoint of its pulled populations (own) and,
-// for a corner, of the adjacent wall node's pulled populations (nbr).  The
-// boundary map is affine in the pulled populations, so its columns are
-// exact differences of two evaluations.
+// for a corner, of the adjacent wall node's pulled populations.  The boundary
+// map is affine in the pulled populations, so its columns are exact
+// differences of two evaluations.
+// Column k of that transposed map: sum_q hbar_q d(out_q)/d(in_k), with `in`
+// the node's own pulled populations or, for a corner (nbr), those of the
+// adjacent wall node.
 template <typename T>
-__device__ void bc_transpose(const PbbParams &p, int x, int y, const T *hbar, T *own, T *nbr)
+__device__ T bc_column(const PbbParams &p, int x, int y, const T *hbar, int k, bool nbr)
 {
     const int lx = p.nx - 1, ly = p.ny - 1;
     const bool corner = (x == 0 || x == lx) && (y == 0 || y == ly);
@@ -280,38 +283,21 @@ __device__ void bc_transpose(const PbbParams &p, int x, int y, const T *hbar, T 
             zh_corner(out, which, wall_rho<T>(y, ly, fnb), T(0), T(0));
         }
     };
+    const int xs = nbr ? (x == 0 ? 1 : lx - 1) : x;
+    if (!in_domain(p, xs - CX[k], y - CY[k])) return T(0);   // not pulled: no adjoint
     T zero[9], e[9], base[9], col[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) zero[q] = T(0);
+    for (int q = 0; q < 9; ++q) {
+        zero[q] = T(0);
+        e[q] = q == k ? T(1) : T(0);
+    }
     eval(zero, zero, base);
+    if (nbr) eval(zero, e, col);
+    else eval(e, zero, col);
+    T acc = T(0);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        own[k] = T(0);
-        nbr[k] = T(0);
-    }
-    for (int k = 0; k < 9; ++k) {
-        if (!in_domain(p, x - CX[k], y - CY[k])) continue;   // not pulled: no adjoint
-#pragma unroll
-        for (int q = 0; q < 9; ++q) e[q] = q == k ? T(1) : T(0);
-        eval(e, zero, col);
-        T acc = T(0);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) acc += hbar[q] * (col[q] - base[q]);
-        own[k] = acc;
-    }
-    if (corner) {
-        const int xn = x == 0 ? 1 : lx - 1;
-        for (int k = 0; k < 9; ++k) {
-            if (!in_domain(p, xn - CX[k], y - CY[k])) continue;
-#pragma unroll
-            for (int q = 0; q < 9; ++q) e[q] = q == k ? T(1) : T(0);
-            eval(zero, e, col);
-            T acc = T(0);
-#pragma unroll
-            for (int q = 0; q < 9; ++q) acc += hbar[q] * (col[q] - base[q]);
-            nbr[k] = acc;
-        }
-    }
+    for (int q = 0; q < 9; ++q) acc += hbar[q] * (col[q] - base[q]);
+    return acc;
 }
 
 // d(boundary output)/d(ux, uy) . hbar on a left-wall node (zh_left with
@@ -330,71 +316,95 @@ __device__ __forceinline__ void left_velocity_adjoint(const T *f, T ux, T uy, co
     gy = hbar[5] * (T(K3) * r) - hbar[8] * (T(K3) * r);
 }
 
-// Full adjoint of one node: the node's hbar after the boundary transpose,
-// including, on a wall node next to a domain corner, the corner's term.
+// Interior nodes (every pull in the domain, no boundary map): a lean kernel,
+// so that the perimeter's boundary transposes do not set its register count.
 template <typename T>
-__device__ void node_lambda(const PbbParams &p, int x, int y, int cell, T *out, double &sbar,
-                            bool accumulate)
+__global__ __launch_bounds__(256) void k_pbb_adj_in(PbbParams p)
 {
+    const int cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= p.N) return;
+    const int x = nodex(p, cell), y = cell - x * p.ny;
+    if (x == 0 || x == p.nx - 1 || y == 0 || y == p.ny - 1) return;
+    T h[9], lam[9], hb[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+        h[q] = ld<T>(p.src, (size_t)q * p.N + (size_t)(x - CX[q]) * p.ny + (y - CY[q]));
+#pragma unroll
+    for (int q = 0; q < 9; ++q) lam[q] = ld<T>(p.lam_in, (size_t)q * p.N + cell);
+    const T sv = ld<T>(p.s, cell);
+    double sb;
+    node_adjoint<T>(p, h, sv, lam, hb, sb);
+    p.s_bar[cell] += sb;
+    T *lo = static_cast<T *>(p.lam_out);
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+        lo[(size_t)q * p.N + (size_t)(x - CX[q]) * p.ny + (y - CY[q])] = hb[q];
+}
+
+// Perimeter node i (bottom row, top row, left and right columns without the
+// corners) -> (x, y).
+__device__ __forceinline__ void pbb_perim_node(const PbbParams &p, int i, int &x, int &y)
+{
+    if (i < p.nx) { x = i; y = 0; return; }
+    i -= p.nx;
+    if (i < p.nx) { x = i; y = p.ny - 1; return; }
+    i -= p.nx;
+    if (i < p.ny - 2) { x = 0; y = i + 1; return; }
+    i -= p.ny - 2;
+    x = p.nx - 1;
+    y = i + 1;
+}
+
+// Perimeter nodes: boundary transposes, corner terms, inlet sensitivity.
+// One work-item per (perimeter node, plane k): each recomputes the node's
+// adjoint and evaluates one column of its boundary map, so the nine columns
+// run in parallel instead of in one long dependent chain per node.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pbb_adj_edge(PbbParams p)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int np = 2 * p.nx + 2 * p.ny - 4;
+    if (i >= 9 * np) return;
+    const int node = i / 9, k = i - 9 * node;
+    int x, y;
+    pbb_perim_node(p, node, x, y);
+    const int cell = x * p.ny + y;
     const int lx = p.nx - 1, ly = p.ny - 1;
     T h[9], lam[9], hb[9];
     node_h<T>(p, x, y, h);
 #pragma unroll
     for (int q = 0; q < 9; ++q) lam[q] = ld<T>(p.lam_in, (size_t)q * p.N + cell);
-    const T sv = ld<T>(p.s, cell);
-    node_adjoint<T>(p, h, sv, lam, hb, sbar);
-    const bool interior = x > 0 && x < lx && y > 0 && y < ly;
-    if (interior) {
-#pragma unroll
-        for (int q = 0; q < 9; ++q) out[q] = hb[q];
-        return;
-    }
-    T nbr[9];
-    bc_transpose<T>(p, x, y, hb, out, nbr);
-    if (accumulate && x == 0 && y > 0 && y < ly) {
-        T f[9];
-        pull_full<T>(p, p.src, x, y, f);
-        T gx, gy;
-        left_velocity_adjoint<T>(f, T(p.amp * p.prof[y]), T(p.amp * p.prof[p.ny + y]), hb, gx,
-                                 gy);
-        p.a_bar[y] += p.ramp * ((double)gx * p.prof[y] + (double)gy * p.prof[p.ny + y]);
+    double sb;
+    node_adjoint<T>(p, h, ld<T>(p.s, cell), lam, hb, sb);
+    T out = bc_column<T>(p, x, y, hb, k, false);
+    if (k == 0) {
+        p.s_bar[cell] += sb;
+        if (x == 0 && y > 0 && y < ly) {
+            T f[9];
+            pull_full<T>(p, p.src, x, y, f);
+            T gx, gy;
+            left_velocity_adjoint<T>(f, T(p.amp * p.prof[y]), T(p.amp * p.prof[p.ny + y]), hb,
+                                     gx, gy);
+            p.a_bar[y] += p.ramp * ((double)gx * p.prof[y] + (double)gy * p.prof[p.ny + y]);
+        }
     }
     // a wall node next to a corner also carries the corner's dependence on it
-    const bool bottom_top = (y == 0 || y == ly) && (x == 1 || x == lx - 1);
-    if (bottom_top) {
+    if ((y == 0 || y == ly) && (x == 1 || x == lx - 1)) {
         const int xc = x == 1 ? 0 : lx;
         const int cc = xc * p.ny + y;
-        T hc[9], lc[9], hbc[9], own_c[9], nbr_c[9];
+        T hc[9], lc[9], hbc[9];
         node_h<T>(p, xc, y, hc);
 #pragma unroll
         for (int q = 0; q < 9; ++q) lc[q] = ld<T>(p.lam_in, (size_t)q * p.N + cc);
         double sdummy;
         node_adjoint<T>(p, hc, ld<T>(p.s, cc), lc, hbc, sdummy);
-        bc_transpose<T>(p, xc, y, hbc, own_c, nbr_c);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) out[q] += nbr_c[q];
+        out += bc_column<T>(p, xc, y, hbc, k, true);
     }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_pbb_adj(PbbParams p)
-{
-    const int cell = blockIdx.x * 256 + threadIdx.x;
-    if (cell >= p.N) return;
-    const int x = nodex(p, cell), y = cell - x * p.ny;
-    T hb[9];
-    double sb;
-    node_lambda<T>(p, x, y, cell, hb, sb, true);
-    p.s_bar[cell] += sb;
     T *lo = static_cast<T *>(p.lam_out);
-#pragma unroll
-    for (int q = 0; q < 9; ++q) {
-        const int sx = x - CX[q], sy = y - CY[q];
-        if (in_domain(p, sx, sy)) lo[(size_t)q * p.N + (size_t)sx * p.ny + sy] = hb[q];
-        // entries of this node that no node pulls carry no adjoint
-        const int tx = x + CX[q], ty = y + CY[q];
-        if (!in_domain(p, tx, ty)) lo[(size_t)q * p.N + cell] = T(0);
-    }
+    const int sx = x - CX[k], sy = y - CY[k];
+    if (in_domain(p, sx, sy)) lo[(size_t)k * p.N + (size_t)sx * p.ny + sy] = out;
+    // an entry of this node that no node pulls carries no adjoint
+    if (!in_domain(p, x + CX[k], y + CY[k])) lo[(size_t)k * p.N + cell] = T(0);
 }
 
 __global__ void k_pbb_sum_rows(const double *part, int nb, int rows, double *out)
@@ -483,8 +493,14 @@ void fwd(const lbm_pbb *h, const PbbParams &p, hipStream_t s)
 
 void adj(const lbm_pbb *h, const PbbParams &p, hipStream_t s)
 {
-    if (h->storage == LBM_STORE_F64) k_pbb_adj<double><<<h->nblocks, 256, 0, s>>>(p);
-    else k_pbb_adj<float><<<h->nblocks, 256, 0, s>>>(p);
+    const int eb = (9 * (2 * h->nx + 2 * h->ny - 4) + 255) / 256;
+    if (h->storage == LBM_STORE_F64) {
+        k_pbb_adj_in<double><<<h->nblocks, 256, 0, s>>>(p);
+        k_pbb_adj_edge<double><<<eb, 256, 0, s>>>(p);
+    } else {
+        k_pbb_adj_in<float><<<h->nblocks, 256, 0, s>>>(p);
+        k_pbb_adj_edge<float><<<eb, 256, 0, s>>>(p);
+    }
 }
 
 // state buffer holding H_t during the forward sweep
