@@ -389,8 +389,8 @@ def section_batch(iterations, log):
 
 
 def section_adjoint(log):
-    """C5: 1024x256 channel, solid-fraction disk + noise, 2000 iterations,
-    loss over the last 1000, checkpoints every 100; FP32 forward/adjoint."""
+    """Adjoint: 1024x256 channel, solid-fraction disk + noise, 2000 iterations,
+    loss over the last 1000, checkpoints every 100; FP64 and FP32."""
     import torch
     from lbm_amd.adjoint import PbbEngine, cd_scale, cylinder_case
     case = cylinder_case()
@@ -526,8 +526,9 @@ def main():
     ap.add_argument("--ny", type=int, default=2048)
     ap.add_argument("--cpu-iterations", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sections", default="c3,c1,c2,c4,c5",
-                    help="c3 obstacle array (headline), c1 cavity, c2 turek, c4 batch, c5 adjoint")
+    ap.add_argument("--sections", default="array,cavity,turek,batch,adjoint",
+                    help="array (8192x2048 obstacle array, the headline), cavity (200x200), "
+                         "turek (cylinder), batch (256 cavities), adjoint")
     ap.add_argument("--turek-iterations", type=int, default=200000)
     ap.add_argument("--batch-iterations", type=int, default=5000)
     ap.add_argument("--summarize-profile", metavar="DIR",
@@ -537,8 +538,8 @@ def main():
         summarize_profile(args.summarize_profile, nodes=8192 * args.ny)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the domain decomposition is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (the lattice would be "
+              "split into x-slabs with one halo column per cut; not built)", file=sys.stderr)
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -600,14 +601,14 @@ def main():
         print(msg, file=sys.stderr, flush=True)
 
     extra = {}
-    if "c1" in sections:
+    if "cavity" in sections:
         extra["cavity"] = section_cavity(20001, log)
-    if "c2" in sections:
+    if "turek" in sections:
         extra["turek"] = section_turek(["f64", "f32", "f16", "f16e", "bf16"],
                                        args.turek_iterations, log)
-    if "c4" in sections:
+    if "batch" in sections:
         extra["batch"] = section_batch(args.batch_iterations, log)
-    if "c5" in sections:
+    if "adjoint" in sections:
         extra["adjoint"] = section_adjoint(log)
     line = {
         "metric": "D2Q9 MLUPS (FP16-offset storage, 8192x2048 obstacle array) and % of HBM roofline",
