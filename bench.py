@@ -449,8 +449,8 @@ def summarize_profile(pdir, nodes, copy_bytes=2 << 30):
         return out
 
     def form_of(name):
-        m = re.search(r"k_step(_v4)?<(\d), (\d)>", name)
-        return (FORM_OF_STORE[m.group(2)], m.group(1) is not None, int(m.group(3))) if m else None
+        m = re.search(r"k_step(_v4|_w7)?<(\d), (\d)>", name)
+        return (FORM_OF_STORE[m.group(2)], m.group(1) == "_v4", int(m.group(3))) if m else None
 
     stats = {r["Name"]: r for r in rows("trace", "kernel_stats.csv")}
     counters = {}
@@ -524,7 +524,7 @@ def main():
     ap.add_argument("--vector-width", type=int, default=0,
                     help="16-bit forms: 4 nodes per work-item (4), one (1) or automatic (0)")
     ap.add_argument("--ny", type=int, default=2048)
-    ap.add_argument("--cpu-iterations", type=int, default=40)
+    ap.add_argument("--cpu-iterations", type=int, default=160)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sections", default="array,cavity,turek,batch,adjoint",
                     help="array (8192x2048 obstacle array, the headline), cavity (200x200), "
