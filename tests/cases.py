@@ -213,10 +213,11 @@ def array_oracle_case(gpu_out, iterations=80):
     return dict(rho=rho, u=u, g=g, g_up=gup, drag=np.array(drag))
 
 
-def one_step_pair(storage="f16", cpt=1, ny=64, nx=256, seed=5, iterations=2):
-    """Two FusedDomains of the same channel (no obstacles), 4-node and 1-node
-    16-bit kernels, advanced from one random state; returns the g_up of the
-    last iteration of each (decoded to double)."""
+def one_step_pair(storage="f16", cpt=1, ny=64, nx=256, seed=5, iterations=2, shapes=None):
+    """Two FusedDomains of the same channel (no obstacles), advanced from one
+    random state; returns the g_up of the last iteration of each (decoded to
+    double).  shapes: the two (cells_per_thread, vector_width) launch shapes,
+    by default the 4-node and 1-node 16-bit kernels with cpt nodes."""
     import torch
     from lbm_amd import _lib
     from lbm_amd.domain import FusedDomain, pack_inlets
@@ -229,9 +230,9 @@ def one_step_pair(storage="f16", cpt=1, ny=64, nx=256, seed=5, iterations=2):
              np.ones(ny))] * (iterations + 1)
     pool, offs = pack_inlets(rows)
     outs = []
-    for vw in (4, 1):
+    for c, vw in (shapes or ((cpt, 4), (cpt, 1))):
         dom = FusedDomain(nx, ny, 1 / 0.6, 1 / (0.25 / 0.1 + 0.5), bc, storage=storage,
-                          ring=iterations + 2, cells_per_thread=cpt, vector_width=vw)
+                          ring=iterations + 2, cells_per_thread=c, vector_width=vw)
         dom.set_inlets(0, offs, pool)
         dom.init(g0)
         for it in range(1, iterations + 1):

@@ -2,7 +2,9 @@
 the node count is a multiple of 1024*CPT) against the one-node kernel of the
 same storage form -- same float arithmetic, so bit-identical fields and drag --
 and against the FP64 oracle within the 16-bit tolerances; the FP64 fused step
-on the same obstacle-array case is bit-identical to the oracle."""
+on the same obstacle-array case is bit-identical to the oracle.  f16e with
+two and four nodes per work-item against one: bit-identical (the float
+arithmetic of a node does not depend on the launch shape)."""
 import numpy as np
 import pytest
 
@@ -72,3 +74,27 @@ def test_vector_kernel_random_state(form):
         print(f"case {form} iterations={its}", flush=True)
         v, s = one_step_pair(storage=form, iterations=its)
         assert np.array_equal(v, s), its
+
+
+@pytest.mark.parametrize("cpt", [2, 4])
+def test_f16e_launch_shapes_agree(cpt):
+    print(f"case f16e cpt={cpt} obstacle array", flush=True)
+    a = array_domain_case("f16e", IT, cpt=cpt)
+    b = array_domain_case("f16e", IT, cpt=1)
+    for k in ("rho", "u", "g", "g_up"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(a["drag"], b["drag"])
+
+
+@pytest.mark.parametrize("nx,ny", [(256, 64), (100, 50), (130, 77)])
+def test_f16e_launch_shapes_agree_random_state(nx, ny):
+    """Rows that do not start at a wave boundary (ny = 50, 77) put row ends
+    inside a wave's run and the lattice end inside a partial tile."""
+    for its in (1, 2):
+        print(f"case f16e {nx}x{ny} iterations={its}", flush=True)
+        a, b = one_step_pair(storage="f16e", nx=nx, ny=ny, iterations=its,
+                             shapes=((4, 1), (1, 1)))
+        assert np.array_equal(a, b), its
+        a2, b2 = one_step_pair(storage="f16e", nx=nx, ny=ny, iterations=its,
+                               shapes=((2, 1), (1, 1)))
+        assert np.array_equal(a2, b2), its
