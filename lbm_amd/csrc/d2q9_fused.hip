@@ -653,7 +653,7 @@ template <bool COLS>
 __device__ __forceinline__ void perim_node(const Params &p, int i, int &x, int &y)
 {
     const int nx = p.nx, ny = p.ny;
-    if constexpr (COLS) {
+    if (COLS && p.n_perim == 2 * ny) {   // 16-bit forms: ny >= 64 only
         x = i < ny ? 0 : nx - 1;
         y = i < ny ? i : i - ny;
         return;
@@ -685,28 +685,72 @@ __device__ __forceinline__ void step_perimeter(const Params &p, int pblock, int 
     collide_any<S>(p, cell, f, macro);
 }
 
-// Forms whose one-node tiles complete their bottom and top wall nodes.  On
-// the 8192x2048 obstacle array this takes FP64 from 491 to 460 us and FP32
-// from 256 to 240 us per iteration (the wall nodes' strided 8- and 4-byte
-// accesses of the perimeter work-groups become part of the tile's coalesced
-// ones); the 16-bit forms lose more to the tile's register peak than they
-// gain, so their perimeter work-groups keep every perimeter node.
+// Forms whose one-node tiles complete their bottom and top wall nodes (the
+// 16-bit forms when ny >= 64, see wall_routine; the host then launches
+// perimeter work-groups for the left and right columns only, n_perim = 2ny).
+// On the 8192x2048 obstacle array this takes FP64 from 491 to 460 us, FP32
+// from 256 to 240 us and FP16 from 142 to 138 us per iteration: the wall
+// nodes' strided accesses in the perimeter work-groups become part of the
+// tile's coalesced ones.  f16e keeps every perimeter node in the perimeter
+// work-groups.
 template <int S> constexpr bool walls_in_tile()
 {
-    return S == LBM_STORE_F64 || S == LBM_STORE_F32;
+    return S == LBM_STORE_F64 || S == LBM_STORE_F32 || S == LBM_STORE_F16 ||
+           S == LBM_STORE_BF16;
 }
 
 // Bottom or top wall routine of a tile's wall node (not a corner) on the
 // populations its work-item pulled: hv holds f (FP64) or h = f - w (float
 // forms), where f = w + h is exactly Store::ld.  The float forms go through f
 // and back as the perimeter path does (collide_any), so a wall node completed
-// in a tile rounds exactly as one completed by a perimeter work-group.
+// in a tile rounds exactly as one completed by a perimeter work-group.  The
+// 16-bit forms (ny >= 64: a wave's run of 64 consecutive nodes holds at most
+// one bottom and one top wall node, whose rows follow from the run's first
+// node) read the wall velocity through the scalar cache and evaluate both
+// wall routines, selecting the result: this keeps the tile's register peak
+// within 77 VGPRs and every population in registers.
 template <int S>
 __device__ __forceinline__ void wall_routine(const Params &p, int cell, int row, bool bottom,
                                              typename Store<S>::T *hv)
 {
     using T = typename Store<S>::T;
     const int x = node_x(p, cell);
+    if constexpr (S == LBM_STORE_F16 || S == LBM_STORE_BF16) {
+        // wall velocities are read-only during the launch: scalar loads
+        using cd = const __attribute__((address_space(4))) double;
+        using ci = const __attribute__((address_space(4))) int;
+        const int first = __builtin_amdgcn_readfirstlane(cell - (int)(threadIdx.x & 63));
+        const int xs = bottom ? (first + p.ny - 1) / p.ny : (first + 63 - (p.ny - 1)) / p.ny;
+        const int xc = min(max(xs, 0), p.nx - 1);
+        const int side = bottom ? LBM_SIDE_BOTTOM : LBM_SIDE_TOP;
+        double dux, duy;
+        if (p.amp) {
+            const double a = ((cd *)p.amp)[(size_t)row * p.amp_stride + side];
+            dux = a * ((cd *)p.prof[side])[xc];
+            duy = a * ((cd *)p.prof[side])[p.nx + xc];
+        } else {
+            const int off = ((ci *)p.inlet_off)[row * 5 + side];
+            dux = ((cd *)p.pool)[off + xc];
+            duy = ((cd *)p.pool)[off + p.nx + xc];
+        }
+        // f-space Zou-He exactly as the perimeter path (zh_bottom / zh_top on
+        // f = w + h, then h = f - w), both evaluated and selected so that no
+        // store goes through a selected element
+        const bool apply = bottom ? p.bc_bottom : p.bc_top;
+        const float ux = (float)dux, uy = (float)duy;
+        float f0[9], fb[9], ft[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) f0[q] = fb[q] = ft[q] = weight<float>(q) + hv[q];
+        float rb, rt;
+        zh_bottom(fb, ux, uy, rb);
+        zh_top(ft, ux, uy, rt);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const float fn = apply ? (bottom ? fb[q] : ft[q]) : f0[q];
+            hv[q] = fn - weight<float>(q);
+        }
+        return;
+    }
     T f[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
@@ -760,8 +804,8 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
                 (unsigned)(y - 1) < (unsigned)(ly - 1);
         wall[c] = false;
         if constexpr (walls_in_tile<S>())
-            wall[c] = cell < p.N && (unsigned)(x - 1) < (unsigned)(lx - 1) &&
-                      (y == 0 || y == ly);
+            wall[c] = p.n_perim == 2 * p.ny && cell < p.N &&
+                      (unsigned)(x - 1) < (unsigned)(lx - 1) && (y == 0 || y == ly);
         bottom[c] = y == 0;
         const int lc = (in[c] || wall[c]) ? cell : p.ny + 1;
 #pragma unroll
@@ -1485,9 +1529,12 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     h->n_perim = 2 * d->nx + 2 * d->ny - 4;
     // nodes of the perimeter work-groups: the left and right columns (corners
     // included) where the one-node tiles complete their bottom and top wall
-    // nodes (walls_in_tile), every perimeter node otherwise
+    // nodes (walls_in_tile; 16-bit forms with ny >= 64), every perimeter node
+    // otherwise
     const bool tile_walls = !h->vec4 && (d->storage == LBM_STORE_F64 ||
-                                         d->storage == LBM_STORE_F32);
+                                         d->storage == LBM_STORE_F32 ||
+                                         ((d->storage == LBM_STORE_F16 ||
+                                           d->storage == LBM_STORE_BF16) && d->ny >= 64));
     h->n_perim_wg = tile_walls ? 2 * d->ny : h->n_perim;
     h->magic = (uint32_t)((1ULL << 32) / (uint64_t)d->ny) + 1u;
     h->om_p = d->om_p;
