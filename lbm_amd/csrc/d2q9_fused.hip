@@ -791,11 +791,17 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
     using T = typename St::T;
     using s_t = typename St::s_t;
     const s_t *__restrict__ src = static_cast<const s_t *>(p.src);
-    const int slot = p.tile_slot ? p.tile_slot[tile] : -1;
+    // link tables are read-only during the launch: the tile's slot and each
+    // wave's 64-bit node mask come through the scalar cache with the pull loads
+    using cu32 = const __attribute__((address_space(4))) uint32_t;
+    const int slot = p.tile_slot ? ((const __attribute__((address_space(4))) int *)p.tile_slot)[tile]
+                                 : -1;
     const int base = tile * (256 * CPT) + threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int lx = p.nx - 1, ly = p.ny - 1;
     T v[CPT][9];
     bool in[CPT], wall[CPT], bottom[CPT];
+    uint64_t lmask[CPT];
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
         const int cell = base + c * 256;
@@ -807,6 +813,13 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
             wall[c] = p.n_perim == 2 * p.ny && cell < p.N &&
                       (unsigned)(x - 1) < (unsigned)(lx - 1) && (y == 0 || y == ly);
         bottom[c] = y == 0;
+        lmask[c] = 0;
+        if (S != LBM_STORE_F64 && slot >= 0) {
+            // nodes [first, first + 64) of the wave: two mask words
+            const int lf = (int)((threadIdx.x & ~63u) + c * 256);
+            const cu32 *mw = (const cu32 *)p.cell_mask + (size_t)slot * (8 * CPT) + (lf >> 5);
+            lmask[c] = (uint64_t)mw[0] | ((uint64_t)mw[1] << 32);
+        }
         const int lc = (in[c] || wall[c]) ? cell : p.ny + 1;
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
@@ -825,15 +838,12 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
         if constexpr (!walls_in_tile<S>()) {
             if (!in[c]) continue;
             const int cell = base + c * 256;
-            if (slot >= 0) {
-                const int local = cell & ((256 * CPT) - 1);
-                const uint32_t word = p.cell_mask[(size_t)slot * (8 * CPT) + (local >> 5)];
-                if ((word >> (local & 31)) & 1u) {
-                    int lo, hi;
-                    link_range(p, cell, lo, hi);
-                    apply_links_h<S>(p, cell, lo, hi, v[c]);
-                    store_drag_range_h<S>(p, cell, row, lo, hi, v[c]);
-                }
+            if ((lmask[c] >> lane) & 1u) {
+                const int *off = p.cell_off + (size_t)slot * (256 * CPT + 1) +
+                                 (cell & (256 * CPT - 1));
+                const int lo = off[0], hi = off[1];
+                apply_links_h<S>(p, cell, lo, hi, v[c]);
+                store_drag_range_h<S>(p, cell, row, lo, hi, v[c]);
             }
             collide_store_h<S>(p, cell, v[c], macro);
             continue;
@@ -854,16 +864,24 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
             }
         }
         int lo = 0, hi = 0;
-        if (slot >= 0) {
-            const int local = cell & ((256 * CPT) - 1);
-            const uint32_t word = p.cell_mask[(size_t)slot * (8 * CPT) + (local >> 5)];
-            if ((word >> (local & 31)) & 1u) {
-                link_range(p, cell, lo, hi);
-                if constexpr (S == LBM_STORE_F64)
+        if constexpr (S == LBM_STORE_F64) {
+            // FP64: the node's mask word per work-item (measured faster here
+            // than the wave's scalar mask: 461 against 479 us)
+            if (slot >= 0) {
+                const int local = cell & ((256 * CPT) - 1);
+                const uint32_t word = p.cell_mask[(size_t)slot * (8 * CPT) + (local >> 5)];
+                if ((word >> (local & 31)) & 1u) {
+                    link_range(p, cell, lo, hi);
                     apply_links<S>(p, cell, lo, hi, v[c]);
-                else
-                    apply_links_h<S>(p, cell, lo, hi, v[c]);
+                }
             }
+        } else if ((lmask[c] >> lane) & 1u) {
+            // the node's links: its range in the tile's offset table
+            const int *off = p.cell_off + (size_t)slot * (256 * CPT + 1) +
+                             (cell & (256 * CPT - 1));
+            lo = off[0];
+            hi = off[1];
+            apply_links_h<S>(p, cell, lo, hi, v[c]);
         }
         if (wall[c]) wall_routine<S>(p, cell, row, bottom[c], v[c]);
         if (lo < hi) {
