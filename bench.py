@@ -390,7 +390,9 @@ def section_batch(iterations, log):
 
 def section_adjoint(log):
     """Adjoint: 1024x256 channel, solid-fraction disk + noise, 2000 iterations,
-    loss over the last 1000, checkpoints every 100; FP64 and FP32."""
+    loss over the last 1000, checkpoints every 100; FP64 and FP32.  Forward and
+    backward wall times (host clock, device synchronised) are the medians of
+    three pairs after a warm-up pair."""
     import torch
     from lbm_amd.adjoint import PbbEngine, cd_scale, cylinder_case
     case = cylinder_case()
@@ -405,17 +407,24 @@ def section_adjoint(log):
         eng.forward(s, case["u_max"])           # warm-up
         eng.backward()
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        loss = eng.forward(s, case["u_max"])
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        gs, gu = eng.backward()
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
+        fw, bw = [], []
+        for _ in range(3):                      # medians of three forward/backward pairs
+            t0 = time.perf_counter()
+            loss = eng.forward(s, case["u_max"])
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            gs, gu = eng.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            fw.append(t1 - t0)
+            bw.append(t2 - t1)
+        f_med, b_med = float(np.median(fw)), float(np.median(bw))
         grads[storage] = (loss, gs.cpu().numpy(), gu)
-        out[storage] = dict(loss=loss, cd=loss * cd_scale(case), forward_ms=round((t1 - t0) * 1e3, 2),
-                            backward_ms=round((t2 - t1) * 1e3, 2),
-                            adjoint_to_forward=round((t2 - t1) / (t1 - t0), 3),
+        out[storage] = dict(loss=loss, cd=loss * cd_scale(case), forward_ms=round(f_med * 1e3, 2),
+                            backward_ms=round(b_med * 1e3, 2),
+                            adjoint_to_forward=round(b_med / f_med, 3),
+                            forward_ms_runs=[round(t * 1e3, 2) for t in fw],
+                            backward_ms_runs=[round(t * 1e3, 2) for t in bw],
                             dL_du_max=gu)
         eng.close()
         log(f"# adjoint {storage}: {out[storage]}")

@@ -316,12 +316,10 @@ __device__ __forceinline__ void left_velocity_adjoint(const T *f, T ux, T uy, co
     gy = hbar[5] * (T(K3) * r) - hbar[8] * (T(K3) * r);
 }
 
-// Interior nodes (every pull in the domain, no boundary map): a lean kernel,
-// so that the perimeter's boundary transposes do not set its register count.
+// Interior node (every pull in the domain, no boundary map).
 template <typename T>
-__global__ __launch_bounds__(256) void k_pbb_adj_in(PbbParams p)
+__device__ __forceinline__ void adj_interior_node(const PbbParams &p, int cell)
 {
-    const int cell = blockIdx.x * 256 + threadIdx.x;
     if (cell >= p.N) return;
     const int x = nodex(p, cell), y = cell - x * p.ny;
     if (x == 0 || x == p.nx - 1 || y == 0 || y == p.ny - 1) return;
@@ -360,9 +358,8 @@ __device__ __forceinline__ void pbb_perim_node(const PbbParams &p, int i, int &x
 // adjoint and evaluates one column of its boundary map, so the nine columns
 // run in parallel instead of in one long dependent chain per node.
 template <typename T>
-__global__ __launch_bounds__(256) void k_pbb_adj_edge(PbbParams p)
+__device__ __forceinline__ void adj_perimeter_item(const PbbParams &p, int i)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
     const int np = 2 * p.nx + 2 * p.ny - 4;
     if (i >= 9 * np) return;
     const int node = i / 9, k = i - 9 * node;
@@ -405,6 +402,23 @@ __global__ __launch_bounds__(256) void k_pbb_adj_edge(PbbParams p)
     if (in_domain(p, sx, sy)) lo[(size_t)k * p.N + (size_t)sx * p.ny + sy] = out;
     // an entry of this node that no node pulls carries no adjoint
     if (!in_domain(p, x + CX[k], y + CY[k])) lo[(size_t)k * p.N + cell] = T(0);
+}
+
+// One adjoint iteration in one launch: work-groups [0, eb) take the
+// perimeter items (a long dependent chain each, so they start first and run
+// beside the interior), the rest one interior node per work-item.  The two
+// parts write disjoint entries: lam_out[q][m] is written by the one node that
+// pulls it, s_bar per node, a_bar by perimeter nodes only.  W is an occupancy
+// floor that keeps the interior path's register count (FP64: 4 waves/SIMD,
+// 128 VGPRs; the perimeter path alone would need 158 and spills a little).
+template <typename T, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_pbb_adj(
+    PbbParams p, int eb)
+{
+    if ((int)blockIdx.x < eb)
+        adj_perimeter_item<T>(p, blockIdx.x * 256 + threadIdx.x);
+    else
+        adj_interior_node<T>(p, ((int)blockIdx.x - eb) * 256 + threadIdx.x);
 }
 
 __global__ void k_pbb_sum_rows(const double *part, int nb, int rows, double *out)
@@ -494,13 +508,10 @@ void fwd(const lbm_pbb *h, const PbbParams &p, hipStream_t s)
 void adj(const lbm_pbb *h, const PbbParams &p, hipStream_t s)
 {
     const int eb = (9 * (2 * h->nx + 2 * h->ny - 4) + 255) / 256;
-    if (h->storage == LBM_STORE_F64) {
-        k_pbb_adj_in<double><<<h->nblocks, 256, 0, s>>>(p);
-        k_pbb_adj_edge<double><<<eb, 256, 0, s>>>(p);
-    } else {
-        k_pbb_adj_in<float><<<h->nblocks, 256, 0, s>>>(p);
-        k_pbb_adj_edge<float><<<eb, 256, 0, s>>>(p);
-    }
+    if (h->storage == LBM_STORE_F64)
+        k_pbb_adj<double, 4><<<eb + h->nblocks, 256, 0, s>>>(p, eb);
+    else
+        k_pbb_adj<float, 8><<<eb + h->nblocks, 256, 0, s>>>(p, eb);
 }
 
 // state buffer holding H_t during the forward sweep
