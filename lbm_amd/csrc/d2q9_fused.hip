@@ -653,7 +653,7 @@ template <bool COLS>
 __device__ __forceinline__ void perim_node(const Params &p, int i, int &x, int &y)
 {
     const int nx = p.nx, ny = p.ny;
-    if (COLS && p.n_perim == 2 * ny) {   // 16-bit forms: ny >= 64 only
+    if (COLS && p.n_perim == 2 * ny) {   // 16-bit forms and f16e: ny >= 64 only
         x = i < ny ? 0 : nx - 1;
         y = i < ny ? i : i - ny;
         return;
@@ -689,14 +689,13 @@ __device__ __forceinline__ void step_perimeter(const Params &p, int pblock, int 
 // 16-bit forms when ny >= 64, see wall_routine; the host then launches
 // perimeter work-groups for the left and right columns only, n_perim = 2ny).
 // On the 8192x2048 obstacle array this takes FP64 from 491 to 460 us, FP32
-// from 256 to 240 us and FP16 from 142 to 138 us per iteration: the wall
-// nodes' strided accesses in the perimeter work-groups become part of the
-// tile's coalesced ones.  f16e keeps every perimeter node in the perimeter
-// work-groups.
+// from 256 to 240 us, FP16 from 138 to 133 us and f16e from 206 to 199 us per
+// iteration: the wall nodes' strided accesses in the perimeter work-groups
+// become part of the tile's coalesced ones.
 template <int S> constexpr bool walls_in_tile()
 {
     return S == LBM_STORE_F64 || S == LBM_STORE_F32 || S == LBM_STORE_F16 ||
-           S == LBM_STORE_BF16;
+           S == LBM_STORE_BF16 || S == LBM_STORE_F16E;
 }
 
 // Bottom or top wall routine of a tile's wall node (not a corner) on the
@@ -704,18 +703,18 @@ template <int S> constexpr bool walls_in_tile()
 // forms), where f = w + h is exactly Store::ld.  The float forms go through f
 // and back as the perimeter path does (collide_any), so a wall node completed
 // in a tile rounds exactly as one completed by a perimeter work-group.  The
-// 16-bit forms (ny >= 64: a wave's run of 64 consecutive nodes holds at most
-// one bottom and one top wall node, whose rows follow from the run's first
-// node) read the wall velocity through the scalar cache and evaluate both
-// wall routines, selecting the result: this keeps the tile's register peak
-// within 77 VGPRs and every population in registers.
+// 16-bit forms and f16e (ny >= 64: a wave's run of 64 consecutive nodes
+// holds at most one bottom and one top wall node, whose rows follow from the
+// run's first node) read the wall velocity through the scalar cache and
+// evaluate both wall routines, selecting the result: this keeps every
+// population in registers and the register peak low (FP16: 77 VGPRs).
 template <int S>
 __device__ __forceinline__ void wall_routine(const Params &p, int cell, int row, bool bottom,
                                              typename Store<S>::T *hv)
 {
     using T = typename Store<S>::T;
     const int x = node_x(p, cell);
-    if constexpr (S == LBM_STORE_F16 || S == LBM_STORE_BF16) {
+    if constexpr (S == LBM_STORE_F16 || S == LBM_STORE_BF16 || S == LBM_STORE_F16E) {
         // wall velocities are read-only during the launch: scalar loads
         using cd = const __attribute__((address_space(4))) double;
         using ci = const __attribute__((address_space(4))) int;
@@ -1547,12 +1546,13 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     h->n_perim = 2 * d->nx + 2 * d->ny - 4;
     // nodes of the perimeter work-groups: the left and right columns (corners
     // included) where the one-node tiles complete their bottom and top wall
-    // nodes (walls_in_tile; 16-bit forms with ny >= 64), every perimeter node
-    // otherwise
+    // nodes (walls_in_tile; 16-bit forms and f16e with ny >= 64), every
+    // perimeter node otherwise
     const bool tile_walls = !h->vec4 && (d->storage == LBM_STORE_F64 ||
                                          d->storage == LBM_STORE_F32 ||
                                          ((d->storage == LBM_STORE_F16 ||
-                                           d->storage == LBM_STORE_BF16) && d->ny >= 64));
+                                           d->storage == LBM_STORE_BF16 ||
+                                           d->storage == LBM_STORE_F16E) && d->ny >= 64));
     h->n_perim_wg = tile_walls ? 2 * d->ny : h->n_perim;
     h->magic = (uint32_t)((1ULL << 32) / (uint64_t)d->ny) + 1u;
     h->om_p = d->om_p;
