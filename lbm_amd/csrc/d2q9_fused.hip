@@ -777,6 +777,42 @@ __device__ __forceinline__ void wall_routine(const Params &p, int cell, int row,
     }
 }
 
+// Obstacle links of one wave's run of 64 nodes, link-major (float forms):
+// work-item k takes record r0 + k (r0 + k + 64, ... for runs with more links),
+// loads it, reads the link's sources and writes the link value to the owning
+// node's slot in the wave's LDS area (lv[node][qb - 1], flag bit qb in
+// lf[node]), and the drag term of an interior owner to the ring.  A node with
+// several links thus costs two dependent loads (record, sources) instead of
+// two per link.  Same values as apply_links_h / store_drag_range_h.  All 64
+// work-items of the wave take part (the caller branches on the wave-uniform
+// range); LDS operations of a wave complete in order.
+template <int S>
+__device__ __forceinline__ void link_pass(const Params &p, int r0, int r1, int first, int row,
+                                          float *lv, uint32_t *lf)
+{
+    using St = Store<S>;
+    const auto *src = static_cast<const typename St::s_t *>(p.src);
+    const int lane = threadIdx.x & 63;
+    const int lx = p.nx - 1, ly = p.ny - 1;
+    lf[lane] = 0u;
+    for (int k0 = r0; k0 < r1; k0 += 64) {
+        const int k = k0 + lane;
+        if (k < r1) {
+            const LinkRec L = p.links[k];
+            const float v = link_h<S>(p, L.cell, L);
+            const int o = L.cell - first;
+            lv[o * 8 + (L.qb - 1)] = v;
+            atomicOr(&lf[o], 1u << L.qb);
+            const int x = node_x(p, L.cell), y = L.cell - x * p.ny;
+            if ((unsigned)(x - 1) < (unsigned)(lx - 1) && (unsigned)(y - 1) < (unsigned)(ly - 1)) {
+                const double hA = (double)St::ldh(src, p.N, L.q, L.cell);
+                p.drag_ring[(size_t)row * p.n_links + L.dslot] =
+                    2.0 * weight<double>(L.q) + (hA + (double)v);
+            }
+        }
+    }
+}
+
 // Tile of 256*CPT nodes: every work-item issues the 9*CPT pull loads of its
 // nodes first, then fixes up obstacle-link nodes (tiles with links only),
 // collides and stores.  Bottom and top wall nodes (corners excepted) are
@@ -791,16 +827,17 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
     using s_t = typename St::s_t;
     const s_t *__restrict__ src = static_cast<const s_t *>(p.src);
     // link tables are read-only during the launch: the tile's slot and each
-    // wave's 64-bit node mask come through the scalar cache with the pull loads
-    using cu32 = const __attribute__((address_space(4))) uint32_t;
-    const int slot = p.tile_slot ? ((const __attribute__((address_space(4))) int *)p.tile_slot)[tile]
-                                 : -1;
+    // wave's record range come through the scalar cache with the pull loads
+    using ci32 = const __attribute__((address_space(4))) int;
+    const int slot = p.tile_slot ? ((const ci32 *)p.tile_slot)[tile] : -1;
     const int base = tile * (256 * CPT) + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int lx = p.nx - 1, ly = p.ny - 1;
     T v[CPT][9];
     bool in[CPT], wall[CPT], bottom[CPT];
-    uint64_t lmask[CPT];
+    int rbeg[CPT], rend[CPT];   // float forms: link records of each wave's run
+    __shared__ float s_lv[4][64 * 8];   // link values per wave, node, direction
+    __shared__ uint32_t s_lf[4][64];    // direction bits of the links received
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
         const int cell = base + c * 256;
@@ -812,12 +849,14 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
             wall[c] = p.n_perim == 2 * p.ny && cell < p.N &&
                       (unsigned)(x - 1) < (unsigned)(lx - 1) && (y == 0 || y == ly);
         bottom[c] = y == 0;
-        lmask[c] = 0;
+        rbeg[c] = rend[c] = 0;
         if (S != LBM_STORE_F64 && slot >= 0) {
-            // nodes [first, first + 64) of the wave: two mask words
-            const int lf = (int)((threadIdx.x & ~63u) + c * 256);
-            const cu32 *mw = (const cu32 *)p.cell_mask + (size_t)slot * (8 * CPT) + (lf >> 5);
-            lmask[c] = (uint64_t)mw[0] | ((uint64_t)mw[1] << 32);
+            // nodes [first, first + 64) of the wave: their records are
+            // [cell_off[first], cell_off[first + 64]) of the tile's table
+            const int lf = __builtin_amdgcn_readfirstlane((int)((threadIdx.x & ~63u) + c * 256));
+            const ci32 *co = (const ci32 *)p.cell_off + (size_t)slot * (256 * CPT + 1) + lf;
+            rbeg[c] = co[0];
+            rend[c] = co[64];
         }
         const int lc = (in[c] || wall[c]) ? cell : p.ny + 1;
 #pragma unroll
@@ -834,18 +873,16 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
     }
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
-        if constexpr (!walls_in_tile<S>()) {
-            if (!in[c]) continue;
-            const int cell = base + c * 256;
-            if ((lmask[c] >> lane) & 1u) {
-                const int *off = p.cell_off + (size_t)slot * (256 * CPT + 1) +
-                                 (cell & (256 * CPT - 1));
-                const int lo = off[0], hi = off[1];
-                apply_links_h<S>(p, cell, lo, hi, v[c]);
-                store_drag_range_h<S>(p, cell, row, lo, hi, v[c]);
+        // float forms: the run's links, link-major, before the nodes branch
+        uint32_t lflags = 0;
+        if constexpr (S != LBM_STORE_F64) {
+            if (rend[c] > rbeg[c]) {
+                const int w = threadIdx.x >> 6;
+                link_pass<S>(p, rbeg[c], rend[c],
+                             __builtin_amdgcn_readfirstlane(base + c * 256 - lane), row,
+                             s_lv[w], s_lf[w]);
+                lflags = s_lf[w][lane];
             }
-            collide_store_h<S>(p, cell, v[c], macro);
-            continue;
         }
         if (!in[c] && !wall[c]) continue;
         const int cell = base + c * 256;
@@ -874,13 +911,13 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
                     apply_links<S>(p, cell, lo, hi, v[c]);
                 }
             }
-        } else if ((lmask[c] >> lane) & 1u) {
-            // the node's links: its range in the tile's offset table
-            const int *off = p.cell_off + (size_t)slot * (256 * CPT + 1) +
-                             (cell & (256 * CPT - 1));
-            lo = off[0];
-            hi = off[1];
-            apply_links_h<S>(p, cell, lo, hi, v[c]);
+        } else if (lflags) {
+            const float *mine = s_lv[threadIdx.x >> 6] + lane * 8;
+#pragma unroll
+            for (int q = 1; q < 9; ++q)
+                if ((lflags >> q) & 1u) v[c][q] = mine[q - 1];
+            // a wall node's drag terms follow its wall routine (below)
+            if (wall[c]) link_range(p, cell, lo, hi);
         }
         if (wall[c]) wall_routine<S>(p, cell, row, bottom[c], v[c]);
         if (lo < hi) {
