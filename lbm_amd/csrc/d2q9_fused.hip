@@ -1559,9 +1559,12 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     h->N = d->nx * d->ny;
     h->storage = d->storage;
     h->cpt = d->cells_per_thread;
+    // default nodes per work-item, the fastest on the 8192x2048 obstacle array:
+    // FP16/BF16 4, f16e 1 (176 against 181 us at 4), FP32/FP64 2
     if (h->cpt == 0)
-        h->cpt = (d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16 ||
-                  d->storage == LBM_STORE_F16E) ? 4 : 2;
+        h->cpt = (d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16) ? 4
+                 : d->storage == LBM_STORE_F16E                               ? 1
+                                                                              : 2;
     if (h->cpt != 1 && h->cpt != 2 && h->cpt != 4 && h->cpt != 8)
         return set_error(LBM_ERR_ARG, "cells_per_thread must be 1, 2, 4 or 8");
     // 16-bit forms: four nodes per work-item on request, when the lattice
