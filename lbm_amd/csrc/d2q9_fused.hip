@@ -685,18 +685,13 @@ __device__ __forceinline__ void step_perimeter(const Params &p, int pblock, int 
     collide_any<S>(p, cell, f, macro);
 }
 
-// Forms whose one-node tiles complete their bottom and top wall nodes (the
-// 16-bit forms when ny >= 64, see wall_routine; the host then launches
-// perimeter work-groups for the left and right columns only, n_perim = 2ny).
-// On the 8192x2048 obstacle array this takes FP64 from 491 to 460 us, FP32
-// from 256 to 240 us, FP16 from 138 to 133 us and f16e from 206 to 199 us per
-// iteration: the wall nodes' strided accesses in the perimeter work-groups
-// become part of the tile's coalesced ones.
-template <int S> constexpr bool walls_in_tile()
-{
-    return S == LBM_STORE_F64 || S == LBM_STORE_F32 || S == LBM_STORE_F16 ||
-           S == LBM_STORE_BF16 || S == LBM_STORE_F16E;
-}
+// The one-node tiles complete their bottom and top wall nodes whenever the
+// perimeter work-groups take the left and right columns only (n_perim = 2ny:
+// every form except with the four-node kernel; the 16-bit forms and f16e need
+// ny >= 64, see wall_routine).  On the 8192x2048 obstacle array this took FP64
+// from 491 to 460 us, FP32 from 256 to 240 us, FP16 from 138 to 133 us and
+// f16e from 206 to 199 us per iteration: the wall nodes' strided accesses in
+// the perimeter work-groups become part of the tile's coalesced ones.
 
 // Bottom or top wall routine of a tile's wall node (not a corner) on the
 // populations its work-item pulled: hv holds f (FP64) or h = f - w (float
@@ -844,10 +839,8 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
         const int x = node_x(p, cell), y = cell - x * p.ny;
         in[c] = cell < p.N && (unsigned)(x - 1) < (unsigned)(lx - 1) &&
                 (unsigned)(y - 1) < (unsigned)(ly - 1);
-        wall[c] = false;
-        if constexpr (walls_in_tile<S>())
-            wall[c] = p.n_perim == 2 * p.ny && cell < p.N &&
-                      (unsigned)(x - 1) < (unsigned)(lx - 1) && (y == 0 || y == ly);
+        wall[c] = p.n_perim == 2 * p.ny && cell < p.N &&
+                  (unsigned)(x - 1) < (unsigned)(lx - 1) && (y == 0 || y == ly);
         bottom[c] = y == 0;
         rbeg[c] = rend[c] = 0;
         if (S != LBM_STORE_F64 && slot >= 0) {
@@ -944,7 +937,7 @@ __device__ __forceinline__ void step_launch(const Params &p)
     const int row = bc_row(p);
     const bool macro = p.rho_out != nullptr;
     if ((int)blockIdx.x >= p.n_main)
-        step_perimeter<S, walls_in_tile<S>()>(p, (int)blockIdx.x - p.n_main, row, macro);
+        step_perimeter<S, true>(p, (int)blockIdx.x - p.n_main, row, macro);
     else
         step_tile<S, CPT>(p, xcd_tile(blockIdx.x, p.n_main, p.rev), row, macro);
 }
@@ -1105,7 +1098,7 @@ __global__ __launch_bounds__(256) void k_batch_step(const Params *__restrict__ c
         p.base = base;
         const int row = bc_row(p);
         if (item.y >= p.n_main) {
-            step_perimeter<S, !V4 && walls_in_tile<S>()>(p, item.y - p.n_main, row, false);
+            step_perimeter<S, !V4>(p, item.y - p.n_main, row, false);
         } else {
             if constexpr (V4)
                 step_tile_v4<S, CPT>(p, item.y, row, false);
@@ -1586,7 +1579,7 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     h->n_perim = 2 * d->nx + 2 * d->ny - 4;
     // nodes of the perimeter work-groups: the left and right columns (corners
     // included) where the one-node tiles complete their bottom and top wall
-    // nodes (walls_in_tile; 16-bit forms and f16e with ny >= 64), every
+    // nodes (16-bit forms and f16e: ny >= 64), every
     // perimeter node otherwise
     const bool tile_walls = !h->vec4 && (d->storage == LBM_STORE_F64 ||
                                          d->storage == LBM_STORE_F32 ||
