@@ -66,6 +66,22 @@ def test_vector_kernel_close_to_oracle(oracle_long, form, tol):
     assert np.abs(v["drag"] - o["drag"]).max() <= tol * dref
 
 @pytest.mark.parametrize("form", ["f16", "bf16"])
+def test_vector_kernel_matches_scalar_long_faces(form):
+    """Squares of side 40-60 in a 512x128 channel: runs of 64 nodes along a
+    face hold up to 146 link records, so the one-node kernel's link-major pass
+    takes several rounds; the four-node kernel applies the same links node by
+    node.  Fields and drag bit-identical."""
+    print(f"case {form} 512x128 long faces", flush=True)
+    kw = dict(ny=128, n_obs=2, seed=1, side=(40, 60))
+    v = array_domain_case(form, 60, vector_width=4, cpt=1, **kw)
+    s = array_domain_case(form, 60, vector_width=1, cpt=1, **kw)
+    for k in ("rho", "u", "g", "g_up"):
+        assert np.array_equal(v[k], s[k]), k
+    assert np.array_equal(v["drag"], s["drag"])
+    assert np.abs(s["drag"]).max() > 0
+
+
+@pytest.mark.parametrize("form", ["f16", "bf16"])
 def test_vector_kernel_random_state(form):
     """From a random (non-smooth) state every node's arithmetic is exercised
     with distinct values, so any rounding difference between the two kernel
