@@ -653,7 +653,7 @@ template <bool COLS>
 __device__ __forceinline__ void perim_node(const Params &p, int i, int &x, int &y)
 {
     const int nx = p.nx, ny = p.ny;
-    if (COLS && p.n_perim == 2 * ny) {   // 16-bit forms and f16e: ny >= 64 only
+    if (COLS && p.n_perim == 2 * ny) {   // FP64/FP32; 16-bit forms and f16e if ny >= 64
         x = i < ny ? 0 : nx - 1;
         y = i < ny ? i : i - ny;
         return;
