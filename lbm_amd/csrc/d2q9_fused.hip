@@ -131,15 +131,18 @@ template <> struct Store<LBM_STORE_BF16> {
 // F16E: E_q(v), explicit fmaf only, so every caller rounds identically
 // c_q.v is taken as +-vx, +-vy or their sum (no multiplications by the
 // integer components, which the compiler may not drop without fast-math), and
-// E_q = w_q cu (3 + 4.5 cu) - 1.5 w_q v.v; q may be a run-time value (the link
-// path), the selection then yields the same operations.
+// E_q = cu (3 w_q + 4.5 w_q cu) - 1.5 w_q v.v with the products of the weights
+// folded into constants (three operations after v.v and cu); q may be a
+// run-time value (the link path), the selection then yields the same
+// operations.  Writer and reader evaluate it on the same half-precision
+// velocity, so the shift cancels exactly whatever its own rounding.
 __device__ __forceinline__ float eq_shift(int q, float vx, float vy)
 {
     const float vv = __builtin_fmaf(vy, vy, vx * vx);
     const float ax = CX[q] > 0 ? vx : -vx, ay = CY[q] > 0 ? vy : -vy;
     const float cu = CY[q] == 0 ? (CX[q] == 0 ? 0.0f : ax) : (CX[q] == 0 ? ay : ax + ay);
-    const float t = cu * __builtin_fmaf(4.5f, cu, 3.0f);
-    return __builtin_fmaf(weight<float>(q), t, (-1.5f * weight<float>(q)) * vv);
+    const float w = weight<float>(q);
+    return __builtin_fmaf(cu, __builtin_fmaf(4.5f * w, cu, 3.0f * w), (-1.5f * w) * vv);
 }
 
 __device__ __forceinline__ void vel_unpack(uint32_t v, float &vx, float &vy)
