@@ -934,15 +934,25 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
 // (perimeter nodes of the tile load a valid interior node and are skipped),
 // then fixes up obstacle-link nodes (tiles with links only) and collides.
 // Work-groups [n_main, grid) apply the Zou-He perimeter plan node by node.
+// Work-groups of the one-node step: first the perimeter work-groups (rounded
+// up to a multiple of 8 so that the tiles keep their XCD mapping), so that
+// their node-by-node chains run beside the tiles instead of after them
+// (FP16 on the 8192x2048 obstacle array: 127.9 -> 125.2 us), then the tiles.
+__host__ __device__ constexpr int perim_groups(int n_perim)
+{
+    return (((n_perim + 255) / 256) + 7) & ~7;
+}
+
 template <int S, int CPT>
 __device__ __forceinline__ void step_launch(const Params &p)
 {
     const int row = bc_row(p);
     const bool macro = p.rho_out != nullptr;
-    if ((int)blockIdx.x >= p.n_main)
-        step_perimeter<S, true>(p, (int)blockIdx.x - p.n_main, row, macro);
+    const int np = perim_groups(p.n_perim);
+    if ((int)blockIdx.x < np)
+        step_perimeter<S, true>(p, (int)blockIdx.x, row, macro);
     else
-        step_tile<S, CPT>(p, xcd_tile(blockIdx.x, p.n_main, p.rev), row, macro);
+        step_tile<S, CPT>(p, xcd_tile((int)blockIdx.x - np, p.n_main, p.rev), row, macro);
 }
 
 template <int S, int CPT>
@@ -1306,7 +1316,7 @@ void launch_fused_s(const lbm_fused *h, const Params &p, hipStream_t s)
 
 template <int S> void launch_step_s(const lbm_fused *h, const Params &p, hipStream_t s)
 {
-    const int grid = h->n_tiles + (h->n_perim_wg + 255) / 256;
+    const int grid = h->n_tiles + perim_groups(h->n_perim_wg);
     switch (h->cpt) {
     case 1: k_step<S, 1><<<grid, 256, 0, s>>>(p); break;
     case 2:
