@@ -9,18 +9,23 @@ lbm/src/core/nb.py in d2q9_oracle.c (built by oracle/Makefile into
 oracle/_build/).  ``run`` restates the reference time loop (lbm/src/core/run.py).
 
 Pins of this oracle (each one a check a plausible mistake would fail):
-  * reference golden data: lbm/tst/cavity/cavity_{vx,uy}_ref (all 200 values,
-    1e-6, 100x100 cavity, 10 001 steps) -- covers macro, equilibrium, TRT
-    collision, streaming, all four Zou-He velocity walls and the corners;
+  * reference golden data: lbm/tst/cavity/test_cavity.py's four asserted
+    values (1e-6) and all 200 values of cavity_{vx,uy}_ref (5e-6; the files
+    were written by the fastmath Numba/BLAS build, measured deviation 3.2e-6),
+    100x100 cavity, 10 001 steps -- covers macro, equilibrium, TRT collision,
+    streaming, all four Zou-He velocity walls and the corners;
   * reference test_poiseuille (L1 < 1e-3 against the analytic parabola) --
     covers the parabolic Zou-He inlet and the Zou-He pressure outlet;
   * closed forms and invariants (tests/test_oracle.py): equilibrium moments,
-    collision invariance of rho and rho*u, streaming as a permutation, the
-    rest state as a fixed point, Zou-He walls reproducing the imposed u,
-    IBB at p = 1/2 reducing to plain bounce-back, the drag of a symmetric
-    population field vanishing by symmetry, and a NumPy restatement of each
-    function on small random inputs.
-Parity pinned for every function; none is "parity unpinned".
+    collision invariance of rho and rho*u, streaming along c_q, the rest
+    state as a fixed point, Zou-He walls reproducing the imposed u or
+    density, corners at rest, drag of a single link and of a symmetric body;
+  * IBB (tests/test_oracle_ibb.py): exactness of the quadratic
+    Bouzidi-Firdaouss-Lallemand interpolation on random quadratics for
+    p in {0.1, 0.3, 0.5, 0.7, 0.95} and all eight directions, with unrelated
+    values around the stencil, plus planted plane / offset / direction / sign
+    mistakes in copies of d2q9_oracle.c that this check must catch.
+Every function has such a check; none is "parity unpinned".
 """
 import ctypes
 import math

@@ -7,6 +7,8 @@ equilibrium moments, collision invariants, streaming along c_q, the rest
 state as a fixed point, Zou-He walls imposing their velocity/density, corners
 at rest, IBB preserving a uniform field for every distance p (and reducing to
 plain bounce-back at p = 1/2), drag of a single link and of a symmetric body.
+The IBB stencil itself (planes, offsets, direction) is pinned by the
+quadratic-exactness checks of test_oracle_ibb.py.
 """
 import os
 
