@@ -79,12 +79,33 @@ def inlet_rows(app, n_rows):
     return rows
 
 
-def run_form(app, lat, om_p, om_m, storage, steps, warmup, graph_len, cpt=0, vector_width=0):
+# the obstacle-array configuration runs 10 000 iterations: the accuracy of
+# every form is measured there, whatever the number of timed iterations
+ACC_ITERATIONS = 10000
+
+
+def chunks(it, end, graph_len):
+    """(first iteration, length) of the graph replays covering it..end-1."""
+    out = []
+    while it < end:
+        n = min(graph_len, end - it)
+        out.append((it, n))
+        it += n
+    return out
+
+
+def run_form(app, lat, om_p, om_m, storage, steps, warmup, graph_len, cpt=0, vector_width=0,
+             acc_iterations=ACC_ITERATIONS):
+    """One storage form on the obstacle array: iterations 1..warmup-1 untimed,
+    the next `steps` timed (their graphs captured and uploaded beforehand,
+    so the timed region holds launches only), then on to the configuration's
+    length untimed; the drag of its last 1000 iterations, the final velocity
+    field and the density sum are returned for the accuracy comparison."""
     import torch
     from lbm_amd import _lib
     from lbm_amd.domain import FusedDomain, pack_inlets
     nx, ny = app.nx, app.ny
-    total = warmup + steps
+    total = max(warmup + steps, acc_iterations)    # iterations 0 .. total-1
     bc = [_lib.BC_VELOCITY, _lib.BC_VELOCITY, _lib.BC_VELOCITY, _lib.BC_PRESSURE]
     dom = FusedDomain(nx, ny, om_p, om_m, bc, [(o.boundary, o.ibb) for o in app.obstacles],
                       ibb=True, storage=storage, ring=total + 3, cells_per_thread=cpt,
@@ -95,32 +116,32 @@ def run_form(app, lat, om_p, om_m, storage, steps, warmup, graph_len, cpt=0, vec
     g0 = w[:, None, None].expand(9, nx, ny).contiguous().cuda()
     dom.init(g0)
     it = 1
-    while it < warmup:
-        n = min(graph_len, warmup - it)
-        dom.steps(it, n)
-        it += n
+    for i0, n in chunks(it, max(warmup, 1), graph_len):
+        dom.steps(i0, n)
+    it = max(warmup, 1)
+    timed = chunks(it, it + steps, graph_len)
+    for i0, n in timed:
+        dom.prepare(i0, n)
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     start.record()
-    end_it = it + steps
-    while it < end_it:
-        n = min(graph_len, end_it - it)
-        dom.steps(it, n)
-        it += n
+    for i0, n in timed:
+        dom.steps(i0, n)
     stop.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = start.elapsed_time(stop)
-    last = it - 1
-    # outputs of the run, after the timed region: one more iteration with its
-    # moments stored, the drag of the last 1000 iterations, the final fields
+    it += steps
+    # untimed: on to the configuration's last iteration, with its moments
+    last = total - 1
+    for i0, n in chunks(it, last, graph_len):
+        dom.steps(i0, n)
     rho = torch.empty((nx, ny), dtype=torch.float64, device="cuda")
     u = torch.empty((2, nx, ny), dtype=torch.float64, device="cuda")
-    dom.step(it, rho, u)
-    last = it
+    dom.step(last, rho, u)
     dom.drag_only(last)
-    n_avg = min(1000, steps)
+    n_avg = 1000
     drag = dom.drag_read(last - n_avg + 1, n_avg)
     g = torch.empty((9, nx, ny), dtype=torch.float64, device="cuda")
     gup = torch.empty_like(g)
@@ -133,11 +154,13 @@ def run_form(app, lat, om_p, om_m, storage, steps, warmup, graph_len, cpt=0, vec
     torch.cuda.synchronize()
     mlups = nx * ny * steps / (ms * 1e-3) / 1e6
     return dict(ms=ms, wall=wall, mlups=mlups, drag=drag, u=ux, mass=mass, device_bytes=dbytes,
-                n_links=int(sum(len(o.boundary) for o in app.obstacles)))
+                n_links=int(sum(len(o.boundary) for o in app.obstacles)), iterations=total)
 
 
-def copy_ceiling_gbs(gib=2, blocks=4096, reps=10):
-    """HBM copy ceiling: lbm_copy_probe moving `gib` GiB (read + write bytes / time)."""
+def copy_ceiling(gib=2, reps=10):
+    """HBM copy ceiling: the copy probes (16-byte accesses, plain and
+    non-temporal) moving `gib` GiB with 4096 and 32 768 work-groups (read +
+    write bytes / time); the best configuration is the ceiling."""
     import torch
     from lbm_amd import _lib
     L = _lib.load()
@@ -145,18 +168,23 @@ def copy_ceiling_gbs(gib=2, blocks=4096, reps=10):
     a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
     st = _lib.stream_ptr()
-    for _ in range(3):
-        _lib.check(L.lbm_copy_probe(b.data_ptr(), a.data_ptr(), nbytes, blocks, st))
-    torch.cuda.synchronize()
-    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    s.record()
-    for _ in range(reps):
-        _lib.check(L.lbm_copy_probe(b.data_ptr(), a.data_ptr(), nbytes, blocks, st))
-    e.record()
-    torch.cuda.synchronize()
-    gbs = 2 * nbytes * reps / (s.elapsed_time(e) * 1e-3) / 1e9
+    runs = {}
+    for name, fn in (("k_copy16", L.lbm_copy_probe), ("k_copy16_nt", L.lbm_copy_probe_nt)):
+        for blocks in (4096, 32768):
+            for _ in range(3):
+                _lib.check(fn(b.data_ptr(), a.data_ptr(), nbytes, blocks, st))
+            torch.cuda.synchronize()
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record()
+            for _ in range(reps):
+                _lib.check(fn(b.data_ptr(), a.data_ptr(), nbytes, blocks, st))
+            e.record()
+            torch.cuda.synchronize()
+            runs[f"{name}/{blocks}"] = round(2 * nbytes * reps / (s.elapsed_time(e) * 1e-3) / 1e9,
+                                             1)
     del a, b
-    return gbs
+    best = max(runs, key=runs.get)
+    return dict(GBps=runs[best], best=best, runs=runs)
 
 
 def cpu_baseline(app, lat, om_p, om_m, iterations):
@@ -245,15 +273,15 @@ def run_turek(app, o, prof, storage, iterations):
         amp[it, 0, 4] = app.rho_lbm
     b.set_amplitudes(0, amp)
     b.init()
+    last = iterations - 1
+    plan = chunks(1, last, 1000)
+    for i0, n in plan:
+        b.prepare(i0, n)
     torch.cuda.synchronize()
     s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s0.record()
-    it = 1
-    last = iterations - 1
-    while it < last:
-        n = min(1000, last - it)
-        b.steps(it, n)
-        it += n
+    for i0, n in plan:
+        b.steps(i0, n)
     s1.record()
     torch.cuda.synchronize()
     ms = s0.elapsed_time(s1)
@@ -321,14 +349,14 @@ def section_cavity(iterations, log):
     b = Batch(cases, storage="f64", ring=iterations + 2)
     b.set_amplitudes(0, cavity_amplitudes(cases, 0, iterations + 1))
     b.init()
+    plan = chunks(1, iterations, 1000)
+    for i0, n in plan:
+        b.prepare(i0, n)
     torch.cuda.synchronize()
     s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s0.record()
-    it = 1
-    while it < iterations:
-        n = min(1000, iterations - it)
-        b.steps(it, n)
-        it += n
+    for i0, n in plan:
+        b.steps(i0, n)
     s1.record()
     torch.cuda.synchronize()
     ms = s0.elapsed_time(s1)
@@ -353,15 +381,15 @@ def section_batch(iterations, log):
         b = Batch(cases, storage=storage, ring=iterations + 2, **kw)
         b.set_amplitudes(0, amp)
         b.init()
+        last = iterations - 1
+        plan = chunks(1, last, 1000)
+        for i0, n in plan:
+            b.prepare(i0, n)
         torch.cuda.synchronize()
         s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s0.record()
-        it = 1
-        last = iterations - 1
-        while it < last:
-            n = min(1000, last - it)
-            b.steps(it, n)
-            it += n
+        for i0, n in plan:
+            b.steps(i0, n)
         s1.record()
         torch.cuda.synchronize()
         ms = s0.elapsed_time(s1)
@@ -522,6 +550,29 @@ def summarize_profile(pdir, nodes, copy_bytes=2 << 30):
     print("\n".join(lines))
 
 
+def qualified_form(forms_out, turek, bound=1e-3):
+    """The fastest 16-bit storage form (on the obstacle array) whose Turek
+    max-Cd and max-Cl relative errors against FP64 are both within `bound`."""
+    if not turek:
+        return None
+    best = None
+    for f, e in forms_out.items():
+        t = turek.get(f)
+        if STORE_BYTES[f] != 2 or not t or "max_cd_rel_err" not in t:
+            continue
+        if t["max_cd_rel_err"] <= bound and t["max_cl_rel_err"] <= bound:
+            if best is None or e["mlups"] > forms_out[best]["mlups"]:
+                best = f
+    if best is None:
+        return {"form": None, "bound": bound}
+    e, t = forms_out[best], turek[best]
+    return {"form": best, "bound": bound, "mlups": e["mlups"], "hbm_frac": e["hbm_frac"],
+            "bytes_per_update": e["bytes_per_update"],
+            "speedup_vs_f64": e.get("speedup_vs_f64"),
+            "turek_max_cd_rel_err": t["max_cd_rel_err"],
+            "turek_max_cl_rel_err": t["max_cl_rel_err"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -563,14 +614,17 @@ def main():
         print(f"# form {f}: {res[f]['mlups']:.0f} MLUPS, {res[f]['ms'] / args.steps * 1e3:.1f} "
               f"us/iteration", file=sys.stderr, flush=True)
     ref = res.get("f64")
-    # drag error: time-averaged drag of each obstacle over the last 1000 iterations
+    # accuracy against the FP64 run of the same configuration (10 000
+    # iterations): time-averaged drag of each obstacle over the last 1000
+    # iterations, velocity field of the last iteration
     forms_out = {}
     for f, r in res.items():
         bpu = BYTES_PER_UPDATE[f]
         gbs = r["mlups"] * 1e6 * bpu / 1e9
         entry = {"mlups": round(r["mlups"], 1), "us_per_iteration": round(r["ms"] / args.steps * 1e3, 2),
                  "algorithmic_GBps": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-                 "bytes_per_update": bpu, "mass": r["mass"]}
+                 "bytes_per_update": bpu, "mass": r["mass"],
+                 "accuracy_iterations": r["iterations"]}
         if ref is not None and f != "f64":
             # time-averaged momentum-exchange force of each obstacle over the last
             # iterations; errors relative to the largest FP64 drag of the array
@@ -603,7 +657,7 @@ def main():
                "sample": f"{args.cpu_iterations} iterations of the same 8192x2048 obstacle-array "
                          f"workload, FP64, oracle/d2q9_oracle.c (OpenMP node loops, {threads} "
                          f"threads) driven in the reference's call order ({secs:.1f} s)"}
-    copy_gbs = copy_ceiling_gbs()
+    copy = copy_ceiling()
     sections = set(x.strip() for x in args.sections.split(","))
 
     def log(msg):
@@ -641,11 +695,13 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
-        "copy_ceiling_GBps": round(copy_gbs, 1),
+        "copy_ceiling_GBps": copy["GBps"],
+        "copy_ceiling": copy,
         "cpu_baseline": cpu,
         "forms": forms_out,
     }
     line.update(extra)
+    line["accuracy_qualified"] = qualified_form(forms_out, extra.get("turek"))
     print(json.dumps(line), flush=True)
 
 

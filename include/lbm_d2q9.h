@@ -167,9 +167,12 @@ int  lbm_fused_create(const lbm_fused_desc *desc, lbm_fused **out);
 void lbm_fused_destroy(lbm_fused *h);
 
 /* Inlet table.  Row r holds, for each of the five sides, an offset (in
- * doubles) into a pool of side arrays.  lbm_fused_set_inlets replaces the
- * pool (pool_len doubles) and writes nrows rows starting at row0 (mod ring).
- * Offsets of unused sides may be anything within the pool. */
+ * doubles) into a pool of side arrays: u_bot [2][nx], u_left [2][ny], u_top
+ * [2][nx], u_right [2][ny], rho_right [ny].  lbm_fused_set_inlets replaces
+ * the pool (pool_len doubles) and writes nrows rows starting at row0 (mod
+ * ring).  Every offset must leave its whole side array inside the pool
+ * (offset + length <= pool_len), unused sides included; otherwise nothing is
+ * written and LBM_ERR_ARG is returned. */
 int lbm_fused_set_inlets(lbm_fused *h, int row0, int nrows, const int32_t *offsets_host,
                          int pool_len, const double *pool_host, void *stream);
 
@@ -184,6 +187,12 @@ int lbm_step(lbm_fused *h, int64_t it, double *rho_dev, double *u_dev, void *str
 /* Iterations it0 .. it0+n-1 (it0 >= 1) without moment output, replayed from
  * a captured HIP graph. */
 int lbm_steps(lbm_fused *h, int64_t it0, int n, void *stream);
+
+/* Captures, instantiates and uploads the graph lbm_steps(h, it0, n) replays
+ * (one per length n and parity of it0) without running it, so that a later
+ * lbm_steps call with that key launches only.  No counterpart in the
+ * reference (it has no graphs); for callers that time or capture steps. */
+int lbm_steps_prepare(lbm_fused *h, int64_t it0, int n, void *stream);
 
 /* Stream/boundary/drag half of iteration it (the last one executed):
  * writes the post-boundary populations g (double[9][nx][ny]), the
@@ -260,6 +269,8 @@ int  lbm_batch_set_amplitudes(lbm_batch *b, int64_t row0, int nrows, const doubl
 int  lbm_batch_init(lbm_batch *b, void *stream);
 /* iterations it0 .. it0+n-1 (it0 >= 1) of every case, graph-replayed */
 int  lbm_batch_steps(lbm_batch *b, int64_t it0, int n, void *stream);
+/* the graph of lbm_batch_steps(b, it0, n), captured and uploaded, not run */
+int  lbm_batch_steps_prepare(lbm_batch *b, int64_t it0, int n, void *stream);
 
 /* ------------------------------------------------------------------------
  * Drag sensitivity by a discrete adjoint (no counterpart in the reference).
@@ -309,6 +320,9 @@ int  lbm_pbb_final_state(lbm_pbb *h, void *dst_dev, void *stream);
  * FETCH_SIZE/WRITE_SIZE counters.
  * ---------------------------------------------------------------------- */
 int lbm_copy_probe(void *dst_dev, const void *src_dev, int64_t bytes, int blocks, void *stream);
+/* the same with non-temporal 16-byte loads and stores */
+int lbm_copy_probe_nt(void *dst_dev, const void *src_dev, int64_t bytes, int blocks,
+                      void *stream);
 
 #ifdef __cplusplus
 }

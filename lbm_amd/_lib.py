@@ -100,6 +100,7 @@ _SIGNATURES = {
     "lbm_init": ([c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "lbm_step": ([c_vp, c_i64, c_vp, c_vp, c_vp], c_int),
     "lbm_steps": ([c_vp, c_i64, c_int, c_vp], c_int),
+    "lbm_steps_prepare": ([c_vp, c_i64, c_int, c_vp], c_int),
     "lbm_epilogue": ([c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "lbm_drag_only": ([c_vp, c_i64, c_vp], c_int),
     "lbm_drag_read": ([c_vp, c_i64, c_int, c_vp, c_vp], c_int),
@@ -114,6 +115,7 @@ _SIGNATURES = {
     "lbm_batch_set_amplitudes": ([c_vp, c_i64, c_int, c_vp, c_vp], c_int),
     "lbm_batch_init": ([c_vp, c_vp], c_int),
     "lbm_batch_steps": ([c_vp, c_i64, c_int, c_vp], c_int),
+    "lbm_batch_steps_prepare": ([c_vp, c_i64, c_int, c_vp], c_int),
     "lbm_pbb_create": ([ctypes.POINTER(PbbDesc), ctypes.POINTER(c_vp)], c_int),
     "lbm_pbb_destroy": ([c_vp], None),
     "lbm_pbb_forward": ([c_vp, c_vp, c_dbl, c_vp, c_vp], c_int),
@@ -121,6 +123,7 @@ _SIGNATURES = {
     "lbm_pbb_drag_history": ([c_vp, c_vp], c_int),
     "lbm_pbb_final_state": ([c_vp, c_vp, c_vp], c_int),
     "lbm_copy_probe": ([c_vp, c_vp, c_i64, c_int, c_vp], c_int),
+    "lbm_copy_probe_nt": ([c_vp, c_vp, c_i64, c_int, c_vp], c_int),
 }
 
 _lib = None

@@ -74,6 +74,7 @@ class PbbEngine:
         _lib.check(self.L.lbm_pbb_create(ctypes.byref(d), ctypes.byref(h)), "lbm_pbb_create")
         self.h = h
         self._s = None
+        self.generation = 0          # forwards run so far; a backward pairs with the last
 
     def close(self):
         if getattr(self, "h", None):
@@ -92,7 +93,13 @@ class PbbEngine:
 
     def forward(self, s, u_max):
         """s: CUDA tensor [nx, ny]; returns the loss (float)."""
+        import torch
+        if not isinstance(s, torch.Tensor) or not s.is_cuda:
+            raise TypeError("s must be a CUDA tensor")
+        if tuple(s.shape) != (self.nx, self.ny):
+            raise ValueError(f"s has shape {tuple(s.shape)}, expected {(self.nx, self.ny)}")
         self._s = s.detach().to(self._dtype()).contiguous()
+        self.generation += 1
         out = np.zeros(1)
         _lib.check(self.L.lbm_pbb_forward(self.h, self._s.data_ptr(), float(u_max),
                                           out.ctypes.data_as(_lib.c_vp), _lib.stream_ptr()),
@@ -130,11 +137,16 @@ def _make_function():
         def forward(ctx, s, u_max, engine):
             loss = engine.forward(s, float(u_max))
             ctx.engine = engine
+            ctx.generation = engine.generation
             ctx.s_dtype = s.dtype
             return torch.tensor(loss, dtype=torch.float64, device=s.device)
 
         @staticmethod
         def backward(ctx, grad):
+            # the engine keeps the checkpoints of its last forward only
+            if ctx.engine.generation != ctx.generation:
+                raise RuntimeError("DragLoss.backward: the engine has run another forward "
+                                   "since this loss was computed; recompute the loss first")
             gs, gu = ctx.engine.backward()
             g = grad.to(torch.float64)
             return (g * gs).to(ctx.s_dtype), g * gu, None

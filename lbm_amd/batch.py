@@ -91,6 +91,11 @@ class Batch:
         _lib.check(self.L.lbm_batch_steps(self.h, int(it0), int(n), _lib.stream_ptr()),
                    "lbm_batch_steps")
 
+    def prepare(self, it0, n):
+        """Capture and upload the graph steps(it0, n) will replay, without running it."""
+        _lib.check(self.L.lbm_batch_steps_prepare(self.h, int(it0), int(n), _lib.stream_ptr()),
+                   "lbm_batch_steps_prepare")
+
     def case(self, c):
         """A non-owning FusedDomain view of case c (step, epilogue, mass, drag)."""
         hc = _lib.c_vp()

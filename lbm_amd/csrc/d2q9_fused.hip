@@ -636,7 +636,13 @@ __global__ __launch_bounds__(256) void k_fused(Params p)
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
                 p.g_out[(size_t)q * p.N + cell] = (double)f[q];
-                p.gup_out[(size_t)q * p.N + cell] = (double)St::ld(src, p.N, q, cell);
+                // the float forms' stored populations are w_q + h_q with the
+                // exact double weight w_q
+                if constexpr (S == LBM_STORE_F64)
+                    p.gup_out[(size_t)q * p.N + cell] = St::ld(src, p.N, q, cell);
+                else
+                    p.gup_out[(size_t)q * p.N + cell] =
+                        weight<double>(q) + (double)St::ldh(src, p.N, q, cell);
             }
             if (!interior && p.rho_out) {
                 p.rho_out[cell] = (double)r;
@@ -1157,6 +1163,10 @@ __global__ void k_drag_reduce(const double *__restrict__ ringbuf, int n_links, i
     out[2 * (size_t)t + 1] = fy;
 }
 
+// Density sum of one state buffer in double: FP64 sums f_q; the float forms
+// sum their stored deviations h_q in double (their populations are
+// w_q + h_q with the exact double weights w_q), and k_sum_serial adds
+// N * sum_q w_q once.  Work-group partials land in part[blockIdx.x].
 template <int S>
 __global__ __launch_bounds__(256) void k_mass_partial(const void *buf, int N, double *part)
 {
@@ -1167,7 +1177,12 @@ __global__ __launch_bounds__(256) void k_mass_partial(const void *buf, int N, do
     for (int c = blockIdx.x * 256 + threadIdx.x; c < N; c += gridDim.x * 256) {
         double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < 9; ++q) s += (double)St::ld(src, N, q, c);
+        for (int q = 0; q < 9; ++q) {
+            if constexpr (S == LBM_STORE_F64)
+                s += St::ld(src, N, q, c);
+            else
+                s += (double)St::ldh(src, N, q, c);
+        }
         acc += s;
     }
     sh[threadIdx.x] = acc;
@@ -1179,12 +1194,13 @@ __global__ __launch_bounds__(256) void k_mass_partial(const void *buf, int N, do
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-__global__ void k_sum_serial(const double *part, int n, double *out)
+// out = sum of the n partials + base (the weights' share of the float forms)
+__global__ void k_sum_serial(const double *part, int n, double base, double *out)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     double s = 0.0;
     for (int i = 0; i < n; ++i) s += part[i];
-    out[0] = s;
+    out[0] = s + base;
 }
 
 __global__ void k_set_int(int *p, int v)
@@ -1659,20 +1675,34 @@ extern "C" int lbm_fused_set_inlets(lbm_fused *h, int row0, int nrows, const int
     if (!h || nrows < 0 || nrows > h->ring || pool_len < 0 || (nrows && !off_host) ||
         (pool_len && !pool_host))
         return set_error(LBM_ERR_ARG, "lbm_fused_set_inlets: bad argument");
-    for (int i = 0; i < 5 * nrows; ++i)
-        if (off_host[i] < 0 || (off_host[i] > 0 && off_host[i] >= pool_len))
-            return set_error(LBM_ERR_ARG, "inlet offset outside the pool");
+    // every side array of every row lies inside the pool: u_bot and u_top
+    // [2][nx], u_left and u_right [2][ny], rho_right [ny]
+    const int side_len[5] = {2 * h->nx, 2 * h->ny, 2 * h->nx, 2 * h->ny, h->ny};
+    for (int i = 0; i < 5 * nrows; ++i) {
+        const int64_t o = off_host[i];
+        if (o < 0 || o + side_len[i % 5] > (int64_t)pool_len)
+            return set_error(LBM_ERR_ARG, "inlet row " + std::to_string(i / 5) + ", side " +
+                                              std::to_string(i % 5) + ": offset " +
+                                              std::to_string(o) + " + length " +
+                                              std::to_string(side_len[i % 5]) +
+                                              " exceeds the pool (" + std::to_string(pool_len) +
+                                              " doubles)");
+    }
     // the previous contents may still be read by queued launches
     LBM_HIP_TRY(hipStreamSynchronize(S(stream)));
-    if (pool_len > h->pool_cap) {
+    // rows never written keep offset 0: the allocation holds at least one
+    // side array of the longest kind there (zeros until a pool covers it)
+    const int cap_need = std::max(pool_len, 2 * std::max(h->nx, h->ny));
+    if (cap_need > h->pool_cap) {
         if (h->d_pool) {
             (void)hipFree(h->d_pool);
             h->bytes -= (int64_t)sizeof(double) * h->pool_cap;
         }
         h->d_pool = nullptr;
         h->pool_cap = 0;
-        if (int e = dmalloc(h, &h->d_pool, sizeof(double) * pool_len)) return e;
-        h->pool_cap = pool_len;
+        if (int e = dmalloc(h, &h->d_pool, sizeof(double) * cap_need)) return e;
+        h->pool_cap = cap_need;
+        LBM_HIP_TRY(hipMemset(h->d_pool, 0, sizeof(double) * cap_need));
         for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
         h->graphs.clear();
     }
@@ -1722,11 +1752,11 @@ extern "C" int lbm_step(lbm_fused *h, int64_t it, double *rho, double *u, void *
     return LBM_OK;
 }
 
-extern "C" int lbm_steps(lbm_fused *h, int64_t it0, int n, void *stream)
+namespace {
+// The graph of n steps starting at an iteration of it0's parity, captured
+// and instantiated on first use.
+int step_graph(lbm_fused *h, int64_t it0, int n, hipGraphExec_t *out)
 {
-    if (!h || it0 < 1 || n < 0) return set_error(LBM_ERR_ARG, "lbm_steps: bad argument");
-    if (n == 0) return LBM_OK;
-    if (!h->d_pool && !h->amp) return set_error(LBM_ERR_ARG, "lbm_steps: no inlet table");
     const GraphKey key{n, (int)(it0 & 1)};
     auto itg = h->graphs.find(key);
     if (itg == h->graphs.end()) {
@@ -1753,9 +1783,32 @@ extern "C" int lbm_steps(lbm_fused *h, int64_t it0, int n, void *stream)
             return set_error(LBM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ec));
         itg = h->graphs.emplace(key, exec).first;
     }
+    *out = itg->second;
+    return LBM_OK;
+}
+}  // namespace
+
+extern "C" int lbm_steps(lbm_fused *h, int64_t it0, int n, void *stream)
+{
+    if (!h || it0 < 1 || n < 0) return set_error(LBM_ERR_ARG, "lbm_steps: bad argument");
+    if (n == 0) return LBM_OK;
+    if (!h->d_pool && !h->amp) return set_error(LBM_ERR_ARG, "lbm_steps: no inlet table");
+    hipGraphExec_t exec;
+    if (int e = step_graph(h, it0, n, &exec)) return e;
     k_set_int<<<1, 64, 0, S(stream)>>>(h->d_base, row_of(h, it0 - 1));
     LBM_LAUNCH_CHECK();
-    LBM_HIP_TRY(hipGraphLaunch(itg->second, S(stream)));
+    LBM_HIP_TRY(hipGraphLaunch(exec, S(stream)));
+    return LBM_OK;
+}
+
+extern "C" int lbm_steps_prepare(lbm_fused *h, int64_t it0, int n, void *stream)
+{
+    if (!h || it0 < 1 || n < 0) return set_error(LBM_ERR_ARG, "lbm_steps_prepare: bad argument");
+    if (n == 0) return LBM_OK;
+    if (!h->d_pool && !h->amp) return set_error(LBM_ERR_ARG, "lbm_steps_prepare: no inlet table");
+    hipGraphExec_t exec;
+    if (int e = step_graph(h, it0, n, &exec)) return e;
+    LBM_HIP_TRY(hipGraphUpload(exec, S(stream)));
     return LBM_OK;
 }
 
@@ -1837,6 +1890,17 @@ extern "C" int lbm_fused_restore(lbm_fused *h, int64_t it, void *stream)
     return LBM_OK;
 }
 
+namespace {
+// N * sum_q w_q for the float forms (their partial sums hold the deviations)
+double mass_base(int storage, int N)
+{
+    if (storage == LBM_STORE_F64) return 0.0;
+    double ws = 0.0;
+    for (int q = 0; q < 9; ++q) ws += weight<double>(q);
+    return (double)N * ws;
+}
+}  // namespace
+
 extern "C" int lbm_mass(lbm_fused *h, int64_t it, double *out, void *stream)
 {
     if (!h || it < 0 || !out) return set_error(LBM_ERR_ARG, "lbm_mass: bad argument");
@@ -1854,7 +1918,8 @@ extern "C" int lbm_mass(lbm_fused *h, int64_t it, double *out, void *stream)
     default: k_mass_partial<LBM_STORE_F16E><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
     }
     LBM_LAUNCH_CHECK();
-    k_sum_serial<<<1, 64, 0, S(stream)>>>(h->d_part, nb, h->d_part + nb);
+    k_sum_serial<<<1, 64, 0, S(stream)>>>(h->d_part, nb, mass_base(h->storage, h->N),
+                                          h->d_part + nb);
     LBM_LAUNCH_CHECK();
     LBM_HIP_TRY(hipMemcpyAsync(out, h->d_part + nb, sizeof(double), hipMemcpyDeviceToHost,
                                S(stream)));
@@ -2088,10 +2153,9 @@ extern "C" int lbm_batch_init(lbm_batch *b, void *stream)
     return LBM_OK;
 }
 
-extern "C" int lbm_batch_steps(lbm_batch *b, int64_t it0, int n, void *stream)
+namespace {
+int batch_graph(lbm_batch *b, int64_t it0, int n, hipGraphExec_t *out)
 {
-    if (!b || it0 < 1 || n < 0) return set_error(LBM_ERR_ARG, "lbm_batch_steps: bad argument");
-    if (n == 0) return LBM_OK;
     const GraphKey key{n, (int)(it0 & 1)};
     auto itg = b->graphs.find(key);
     if (itg == b->graphs.end()) {
@@ -2108,9 +2172,31 @@ extern "C" int lbm_batch_steps(lbm_batch *b, int64_t it0, int n, void *stream)
             return set_error(LBM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ec));
         itg = b->graphs.emplace(key, exec).first;
     }
+    *out = itg->second;
+    return LBM_OK;
+}
+}  // namespace
+
+extern "C" int lbm_batch_steps(lbm_batch *b, int64_t it0, int n, void *stream)
+{
+    if (!b || it0 < 1 || n < 0) return set_error(LBM_ERR_ARG, "lbm_batch_steps: bad argument");
+    if (n == 0) return LBM_OK;
+    hipGraphExec_t exec;
+    if (int e = batch_graph(b, it0, n, &exec)) return e;
     const int base = (int)(((it0 - 1) % b->ring + b->ring) % b->ring);
     k_set_int<<<1, 64, 0, S(stream)>>>(b->d_base, base);
     LBM_LAUNCH_CHECK();
-    LBM_HIP_TRY(hipGraphLaunch(itg->second, S(stream)));
+    LBM_HIP_TRY(hipGraphLaunch(exec, S(stream)));
+    return LBM_OK;
+}
+
+extern "C" int lbm_batch_steps_prepare(lbm_batch *b, int64_t it0, int n, void *stream)
+{
+    if (!b || it0 < 1 || n < 0)
+        return set_error(LBM_ERR_ARG, "lbm_batch_steps_prepare: bad argument");
+    if (n == 0) return LBM_OK;
+    hipGraphExec_t exec;
+    if (int e = batch_graph(b, it0, n, &exec)) return e;
+    LBM_HIP_TRY(hipGraphUpload(exec, S(stream)));
     return LBM_OK;
 }

@@ -82,6 +82,12 @@ class FusedDomain:
                                                pool.ctypes.data_as(_lib.c_vp), _lib.stream_ptr()),
                    "lbm_fused_set_inlets")
 
+    def set_inlet_rows(self, row0, rows):
+        """Rows row0.. of the inlet table from snapshots of the five wall
+        arrays, each checked against this lattice's side shapes."""
+        pool, offs = pack_inlets(rows, self.nx, self.ny)
+        self.set_inlets(row0, offs, pool)
+
     def init(self, g, rho=None, u=None):
         _lib.check(self.L.lbm_init(self.h, self._p(g), self._p(rho), self._p(u),
                                    _lib.stream_ptr()), "lbm_init")
@@ -92,6 +98,11 @@ class FusedDomain:
 
     def steps(self, it0, n):
         _lib.check(self.L.lbm_steps(self.h, int(it0), int(n), _lib.stream_ptr()), "lbm_steps")
+
+    def prepare(self, it0, n):
+        """Capture and upload the graph steps(it0, n) will replay, without running it."""
+        _lib.check(self.L.lbm_steps_prepare(self.h, int(it0), int(n), _lib.stream_ptr()),
+                   "lbm_steps_prepare")
 
     def epilogue(self, it, g, g_up, rho=None, u=None):
         _lib.check(self.L.lbm_epilogue(self.h, int(it), self._p(g), self._p(g_up), self._p(rho),
@@ -126,15 +137,30 @@ class FusedDomain:
         return int(self.L.lbm_fused_device_bytes(self.h))
 
 
-def pack_inlets(rows):
+SIDE_NAMES = ("u_bot", "u_left", "u_top", "u_right", "rho_right")
+
+
+def side_shapes(nx, ny):
+    """Shapes of the five wall arrays (lattice.py:164-174)."""
+    return ((2, nx), (2, ny), (2, nx), (2, ny), (ny,))
+
+
+def pack_inlets(rows, nx=None, ny=None):
     """Pool (float64) and offsets [nrows][5] for snapshots of the five wall
     arrays (u_bot, u_left, u_top, u_right, rho_right); identical arrays
-    (same object) are stored once."""
+    (same object) are stored once.  With nx and ny given, every array must
+    have its side's shape (ValueError otherwise)."""
+    want = side_shapes(nx, ny) if nx is not None and ny is not None else None
     pool, offs, where = [np.zeros(1)], [], {}
     n = 1          # offset 0 is a zero pad
-    for snap in rows:
+    for r, snap in enumerate(rows):
+        if len(snap) != 5:
+            raise ValueError(f"inlet row {r}: {len(snap)} arrays, expected 5")
         o = []
-        for a in snap:
+        for sd, a in enumerate(snap):
+            if want is not None and np.shape(a) != want[sd]:
+                raise ValueError(f"inlet row {r}: {SIDE_NAMES[sd]} has shape {np.shape(a)}, "
+                                 f"expected {want[sd]}")
             k = id(a)
             if k not in where:
                 where[k] = n

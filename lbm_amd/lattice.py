@@ -312,6 +312,13 @@ class lattice:
         if self._recording is not None:
             self._recording.append((kind, None))
             return
+        n = self.nx if kind in ("top", "bottom") else self.ny
+        if np.shape(side) != (2, n):
+            raise ValueError(f"zou_he {kind}: wall velocity has shape {np.shape(side)}, "
+                             f"expected {(2, n)}")
+        if extra is not None and np.shape(extra) != (self.ny,):
+            raise ValueError(f"zou_he {kind}: rho_right has shape {np.shape(extra)}, "
+                             f"expected {(self.ny,)}")
         self._check_replay()
         L = self._need_gpu()
         f = self._fields

@@ -17,16 +17,21 @@ the perimeter).  Underneath, iterations run in chunks on the GPU:
    (``lattice.drag_lift`` returns the value of that iteration), check_stop.
 4. If check_stop ends the run inside the chunk, the state is restored from
    the chunk-start checkpoint and re-advanced to the stopping iteration.
+5. Failure detection: after every chunk the FP64 density sum of the state
+   (lbm_mass) must be finite, else InstabilityError is raised; an app
+   attribute ``max_mass_drift`` also bounds its relative drift.  The sums
+   are kept in ``lattice.mass_history``.
 Chunks start at multiples of ``app.output_freq``; an app without it runs one
 iteration per chunk.  A boundary plan the fused kernel cannot express runs the
 reference's call-by-call loop on the per-call kernels instead.
 """
+import math
 import time
 
 import numpy as np
 
 from lbm_amd import _lib
-from lbm_amd.domain import FusedDomain, pack_inlets
+from lbm_amd.domain import FusedDomain
 
 _SIDES = ("u_bot", "u_left", "u_top", "u_right", "rho_right")
 _WALLS = {"bottom": 0, "left": 1, "top": 2, "right": 3, "right_pressure": 3}
@@ -35,6 +40,11 @@ _CORNERS = ("bottom_left", "top_left", "top_right", "bottom_right")
 
 class PlanError(ValueError):
     """The recorded set_bc plan has no fused equivalent."""
+
+
+class InstabilityError(FloatingPointError):
+    """The populations stopped being finite (or the density drifted beyond
+    the app's ``max_mass_drift``) during a fused run."""
 
 
 def record_plan(lattice, app):
@@ -122,6 +132,8 @@ class FusedRun:
                                storage=lattice.precision, ring=self.chunk + 2,
                                cells_per_thread=cells_per_thread)
         self.log = _InletLog(lattice)
+        self.mass_history = []       # (iteration, total density in FP64) per chunk
+        self.max_drift = getattr(app, "max_mass_drift", None)
         self.prev_row = None
         self.drag = None
         self.drag_it0 = 0
@@ -135,8 +147,7 @@ class FusedRun:
     def _upload(self, it0, rows):
         """Inlet rows for iterations it0-1 (repeated from the last chunk) .. it0+len-1."""
         all_rows = ([self.prev_row] if it0 > 0 else []) + rows
-        pool, offs = pack_inlets(all_rows)
-        self.dom.set_inlets(it0 - 1 if it0 > 0 else 0, offs, pool)
+        self.dom.set_inlet_rows(it0 - 1 if it0 > 0 else 0, all_rows)
         self.prev_row = rows[-1]
 
     def _advance(self, it0, it1, macro_last):
@@ -207,6 +218,7 @@ class FusedRun:
             if self.obstacles:
                 dom.drag_only(it1 - 1)
                 self.drag, self.drag_it0 = dom.drag_read(it0, it1 - it0), it0
+            self._check_state(it1 - 1)
             stop_it = self._replay(it0, it1)
             if stop_it is None:
                 it0 = it1
@@ -221,6 +233,24 @@ class FusedRun:
                 f["u"].device(load=False).copy_(self.u_b)
             self._finish(stop_it)
             return stop_it
+
+    def _check_state(self, it):
+        """Failure detection once per chunk: the FP64 density sum of the
+        post-collision state of iteration ``it`` (lbm_mass) must be finite --
+        a NaN or an infinity anywhere in the populations makes it
+        non-finite -- and, if the app sets ``max_mass_drift``, stay within
+        that relative distance of the first chunk's value."""
+        m = self.dom.mass(it)
+        self.mass_history.append((it, m))
+        if not math.isfinite(m):
+            raise InstabilityError(f"non-finite populations after iteration {it} "
+                                   f"(density sum {m})")
+        if self.max_drift is not None:
+            m0 = self.mass_history[0][1]
+            drift = abs(m - m0) / abs(m0)
+            if drift > self.max_drift:
+                raise InstabilityError(f"density drift {drift:.3e} after iteration {it} "
+                                       f"exceeds max_mass_drift = {self.max_drift}")
 
     def _replay(self, it0, it1):
         app, lat = self.app, self.lat
@@ -283,6 +313,7 @@ def run(lattice, app, chunk=1000, fused=True):
         try:
             last = engine.go()
         finally:
+            lattice.mass_history = engine.mass_history
             engine.close()
     else:
         last = _run_per_call(lattice, app)

@@ -95,6 +95,42 @@ def small_array(precision="f64"):
     return app
 
 
+def ref_cavity(precision="f64"):
+    """The reference's default cavity app (100x100, Re = 100, u_lid = 0.2,
+    it_max = 10 000), as lbm/tst/cavity/test_cavity.py runs it."""
+    app = cavity()
+    app.__class__ = type("ref_cavity", (_Quiet, cavity), {})
+    app.precision = precision
+    return app
+
+
+def ref_poiseuille(precision="f64"):
+    """The reference's default Poiseuille app (150x50, it_max = 11 250), as
+    lbm/tst/poiseuille/test_poiseuille.py runs it."""
+    from lbm_amd.app.poiseuille import poiseuille
+    app = poiseuille()
+    app.__class__ = type("ref_poiseuille", (_Quiet, poiseuille), {})
+    app.precision = precision
+    return app
+
+
+def cavity200_app(precision="f64"):
+    """Cavity 200x200, u_lid = 0.1, Re = 100, 20 001 iterations."""
+    from lbm_amd.app.configs import cavity200
+    app = cavity200(precision)
+    app.__class__ = type("q_cavity200", (_Quiet, cavity200), {})
+    return app
+
+
+def turek200_app(precision="f64"):
+    """Schaefer-Turek channel 1073x200 (IBB cylinder D = 48, u_max = 0.05,
+    Re = 100, sigma = 10730), Cd/Cl recorded every iteration."""
+    from lbm_amd.app.configs import turek200
+    app = turek200(precision)
+    app.__class__ = type("q_turek200", (_Quiet, _RecordDrag, turek200), {})
+    return app
+
+
 def _collect(lat, app):
     out = {k: np.array(getattr(lat, k), copy=True) for k in ("rho", "u", "g", "g_up", "g_eq")}
     out["drag"] = np.array(app.drag_series) if hasattr(app, "drag_series") else None
@@ -214,6 +250,57 @@ def array_oracle_case(gpu_out, iterations=80):
             oracle.zou_he(c, u, rho, g)
         drag.append([oracle.drag_sums(o.boundary, gup, g) for o in app.obstacles])
     return dict(rho=rho, u=u, g=g, g_up=gup, drag=np.array(drag))
+
+
+def batch_case_oracle(cs, amp_c, iterations):
+    """One case of a batched ensemble (a Batch case dict) run alone on the
+    oracle: rest state at iteration 0, wall arrays amplitude(it, side) x
+    profile(side), the reference loop of run.py:24-54 with the set_bc order
+    of the reference app of that kind (cavity.py:79-89: bottom, left, right,
+    top; turek.py:112-125: obstacles, bottom, left, top, right), drag sums
+    of every obstacle and iteration."""
+    from lbm_amd import _lib
+    from oracle import oracle
+    nx, ny = int(cs["nx"]), int(cs["ny"])
+    w = np.array([4. / 9.] + [1. / 9.] * 4 + [1. / 36.] * 4)
+    g = np.ascontiguousarray(np.broadcast_to(w[:, None, None], (9, nx, ny)))
+    geq, gup = np.zeros_like(g), np.zeros_like(g)
+    rho, u = np.ones((nx, ny)), np.zeros((2, nx, ny))
+    shapes = ((2, nx), (2, ny), (2, nx), (2, ny), (ny,))
+    profs = cs.get("profiles", [None] * 5)
+    links = cs.get("links")
+    obs = []
+    if links is not None and len(links):
+        off = list(cs.get("obs_offsets", [0, len(links)]))
+        ibb_on = np.asarray(cs.get("ibb_on", np.zeros(len(links))))
+        assert ibb_on.min() == ibb_on.max(), "one IBB flag per case"
+        ibb = bool(ibb_on[0])
+        for a, b in zip(off[:-1], off[1:]):
+            obs.append((np.asarray(links[a:b], np.int64),
+                        np.asarray(cs.get("ibb", np.zeros(len(links)))[a:b], np.float64)))
+    pressure = cs["bc"][3] == _lib.BC_PRESSURE
+    drag = []
+    for it in range(iterations):
+        side = [np.zeros(s) if p is None else amp_c[it, sd] * np.asarray(p, np.float64)
+                for sd, (s, p) in enumerate(zip(shapes, profs))]
+        side = [np.ascontiguousarray(a) for a in side]
+        oracle.macro(g, rho, u)
+        oracle.equilibrium(u, rho, geq)
+        oracle.col_str(g, geq, gup, cs["om_p"], cs["om_m"])
+        for bnd, dist in obs:
+            oracle.bounce_back_obstacle(ibb, bnd, dist, gup, g)
+        oracle.zou_he("bottom", u, rho, g, side=side[0])
+        oracle.zou_he("left", u, rho, g, side=side[1])
+        if pressure:
+            oracle.zou_he("top", u, rho, g, side=side[2])
+            oracle.zou_he("right_pressure", u, rho, g, side=side[3], rho_right=side[4])
+        else:
+            oracle.zou_he("right", u, rho, g, side=side[3])
+            oracle.zou_he("top", u, rho, g, side=side[2])
+        for c in ("bottom_left", "top_left", "top_right", "bottom_right"):
+            oracle.zou_he(c, u, rho, g)
+        drag.append([oracle.drag_sums(bnd, gup, g) for bnd, _ in obs])
+    return dict(rho=rho, u=u, g=g, g_up=gup, drag=np.array(drag) if obs else None)
 
 
 def one_step_pair(storage="f16", cpt=1, ny=64, nx=256, seed=5, iterations=2, shapes=None):

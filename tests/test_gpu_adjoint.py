@@ -10,7 +10,9 @@
   sampled gradient, the FP64 round-off floor of the differences) on the
   1024x256 channel at full size.
 * The gradient does not depend on the checkpoint interval (bit-identical).
-* The FP32 adjoint stays within 2e-2 (relative L2) of the FP64 adjoint.
+* The FP32 adjoint stays within 1e-4 (relative L2) of the FP64 adjoint.
+* A loss whose engine has run another forward refuses its backward; s of
+  the wrong shape or on the host is refused before any launch.
 """
 import numpy as np
 import pytest
@@ -135,7 +137,33 @@ def test_fp32_adjoint_close_to_fp64():
                       window=150, every=40)
     a = _gpu(case, "f64")
     b = _gpu(case, "f32")
-    assert abs(b["loss"] - a["loss"]) <= 1e-3 * abs(a["loss"])
     rel = np.linalg.norm(b["gs"] - a["gs"]) / np.linalg.norm(a["gs"])
-    assert rel <= 2e-2, rel
-    assert abs(b["gu"] - a["gu"]) <= 2e-2 * abs(a["gu"])
+    print(f"f32 vs f64: loss {abs(b['loss'] - a['loss']) / abs(a['loss']):.3e} "
+          f"grad_s {rel:.3e} grad_umax {abs(b['gu'] - a['gu']) / abs(a['gu']):.3e}", flush=True)
+    assert abs(b["loss"] - a["loss"]) <= 1e-5 * abs(a["loss"])
+    assert rel <= 1e-4, rel
+    assert abs(b["gu"] - a["gu"]) <= 1e-4 * abs(a["gu"])
+
+
+def test_backward_of_a_stale_loss_raises():
+    case = small_case(iterations=40, window=20, every=10)
+    eng = PbbEngine(case, storage="f64")
+    s = torch.tensor(case["s"], dtype=torch.float64, device="cuda", requires_grad=True)
+    um = torch.tensor(case["u_max"], dtype=torch.float64, requires_grad=True)
+    first = DragLoss.apply(s, um, eng)
+    second = DragLoss.apply(s, um, eng)
+    with pytest.raises(RuntimeError, match="another forward"):
+        first.backward()
+    second.backward()      # the engine's last forward: fine
+    eng.close()
+
+
+def test_forward_validates_s():
+    case = small_case(iterations=40, window=20, every=10)
+    eng = PbbEngine(case, storage="f64")
+    with pytest.raises(TypeError):
+        eng.forward(torch.tensor(case["s"], dtype=torch.float64), case["u_max"])
+    with pytest.raises(ValueError):
+        eng.forward(torch.zeros((case["nx"] - 1, case["ny"]), dtype=torch.float64,
+                                device="cuda"), case["u_max"])
+    eng.close()

@@ -1,8 +1,13 @@
-"""Batched ensembles (lbm_batch_*): every case of a batch advanced by the
-batched launch equals the same case run alone through the single-domain
-fused step -- bit for bit in FP64 -- and the FP32 batch stays within the
-FP32 tolerance of the FP64 runs.  Cases differ in size (odd sizes included),
-relaxation rates and walls; one carries IBB obstacle links."""
+"""Batched ensembles (lbm_batch_*) against the oracle: every case of a batch
+advanced by the batched launch equals the same case run alone on the oracle
+(the reference loop, run.py:24-54) -- bit for bit in FP64 on rho, u, g, g_up
+and the drag sums -- and the FP32 batch stays within 2e-5 of the oracle's
+velocity scale.  Cases differ in size (odd sizes included), relaxation rates
+and walls; one carries IBB obstacle links.  The 256-cavity ensemble itself
+is run whole and six of its cases (smallest, largest, odd sizes, extreme
+relaxation rates) are compared with the oracle.  The 16-bit forms, which
+have no oracle of their own, are checked for equality between the batched
+and the single-domain launch."""
 import math
 
 import numpy as np
@@ -12,6 +17,7 @@ import torch
 from lbm_amd import _lib
 from lbm_amd.batch import Batch, autotune, cavity_amplitudes, cavity_ensemble
 from lbm_amd.domain import FusedDomain, pack_inlets
+from tests.cases import batch_case_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -56,14 +62,20 @@ def _amp(cases, n):
 
 
 def _final(dom, nx, ny, last):
+    """Last iteration with its moments, the drag sums of iterations 0..last,
+    and the final fields (epilogue)."""
     rho = torch.empty((nx, ny), dtype=torch.float64, device="cuda")
     u = torch.empty((2, nx, ny), dtype=torch.float64, device="cuda")
     g = torch.empty((9, nx, ny), dtype=torch.float64, device="cuda")
     gup = torch.empty_like(g)
     dom.step(last, rho, u)
+    drag = None
+    if dom.n_obs:
+        dom.drag_only(last)
+        drag = dom.drag_read(0, last + 1)
     dom.epilogue(last, g, gup, rho, u)
     return dict(rho=rho.cpu().numpy(), u=u.cpu().numpy(), g=g.cpu().numpy(),
-                g_up=gup.cpu().numpy())
+                g_up=gup.cpu().numpy(), drag=drag)
 
 
 def _single(cs, amp_c, storage, vector_width=0, cpt=0):
@@ -103,15 +115,65 @@ def _batch(cases, amp, storage, **kw):
     return outs
 
 
+def _same(a, b, what):
+    for k in ("rho", "u", "g", "g_up"):
+        if not np.array_equal(a[k], b[k]):
+            d = np.abs(a[k] - b[k])
+            raise AssertionError(f"{what}: {k} differs, max {d.max():.3e}")
+    if b.get("drag") is not None:
+        assert a["drag"] is not None and np.array_equal(a["drag"], b["drag"]), (what, "drag")
+
+
 @pytest.mark.parametrize("kw", [dict(), dict(grid=7), dict(cells_per_thread=1, grid=3)])
-def test_batch_fp64_equals_single_runs(kw):
+def test_batch_fp64_matches_oracle(kw):
+    print(f"case: batch f64 {kw}", flush=True)
     cases = _cases()
     amp = _amp(cases, IT + 1)
     outs = _batch(cases, amp, "f64", **kw)
     for c, cs in enumerate(cases):
-        ref = _single(cs, amp[:, c], "f64")
-        for k in ("rho", "u", "g", "g_up"):
-            assert np.array_equal(outs[c][k], ref[k]), (c, k)
+        ref = batch_case_oracle(cs, amp[:, c], IT)
+        _same(outs[c], ref, f"case {c} ({cs['nx']}x{cs['ny']})")
+
+
+def test_batch_fp32_close_to_oracle():
+    print("case: batch f32", flush=True)
+    cases = _cases()
+    amp = _amp(cases, IT + 1)
+    outs = _batch(cases, amp, "f32")
+    for c, cs in enumerate(cases):
+        ref = batch_case_oracle(cs, amp[:, c], IT)
+        uref = np.abs(ref["u"]).max()
+        assert np.abs(outs[c]["u"] - ref["u"]).max() <= 2e-5 * uref, c
+        assert np.abs(outs[c]["rho"] - ref["rho"]).max() <= 2e-6, c
+
+
+C4_IT = 400
+C4_PICKS = (23, 53, 180, 174, 3, 6)   # n = 65, 511, 499 (Re 993), 479 (tau 1.81), 185, 97
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_c4_ensemble_matches_oracle(storage):
+    """The 256-cavity ensemble (n ~ U{64..512}, Re ~ logU[100, 1000], seed 0)
+    in one batch, 400 iterations of the lid ramp; six cases against the
+    oracle: FP64 bit for bit, FP32 within 2e-5 of the velocity scale."""
+    from lbm_amd.batch import cavity_amplitudes
+    print(f"case: 256-cavity ensemble {storage}, {C4_IT} iterations", flush=True)
+    cases = cavity_ensemble(n_cases=256, seed=0)
+    amp = cavity_amplitudes(cases, 0, C4_IT + 1)
+    b = Batch(cases, storage=storage, ring=C4_IT + 2)
+    b.set_amplitudes(0, amp)
+    b.init()
+    b.steps(1, C4_IT - 2)
+    outs = {c: _final(b.case(c), cases[c]["nx"], cases[c]["ny"], C4_IT - 1) for c in C4_PICKS}
+    b.close()
+    for c in C4_PICKS:
+        ref = batch_case_oracle(cases[c], amp[:, c], C4_IT)
+        if storage == "f64":
+            _same(outs[c], ref, f"ensemble case {c}")
+        else:
+            uref = np.abs(ref["u"]).max()
+            assert np.abs(outs[c]["u"] - ref["u"]).max() <= 2e-5 * uref, c
+            assert np.abs(outs[c]["rho"] - ref["rho"]).max() <= 2e-6, c
 
 
 @pytest.mark.parametrize("storage,vw", [("f16", 1), ("f16", 4), ("bf16", 1), ("bf16", 4),
@@ -144,17 +206,6 @@ def test_batch_16bit_equals_single_runs(storage, vw):
         ref = _single(cs, amp[:, c], storage, vector_width=vw, cpt=1)
         for k in ("rho", "u", "g", "g_up"):
             assert np.array_equal(outs[c][k], ref[k]), (c, k)
-
-
-def test_batch_fp32_close_to_fp64():
-    cases = _cases()
-    amp = _amp(cases, IT + 1)
-    outs = _batch(cases, amp, "f32")
-    for c, cs in enumerate(cases):
-        ref = _single(cs, amp[:, c], "f64")
-        uref = np.abs(ref["u"]).max()
-        assert np.abs(outs[c]["u"] - ref["u"]).max() <= 2e-5 * uref
-        assert np.abs(outs[c]["rho"] - ref["rho"]).max() <= 2e-6
 
 
 def test_autotune_returns_a_valid_shape(tmp_path):

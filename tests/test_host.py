@@ -118,3 +118,37 @@ def test_chunking_follows_output_frequency():
         it = fr._chunk_end(it)
         ends.append(it)
     assert ends == [500, 1000, 1235]
+
+
+def test_pack_inlets_checks_side_shapes():
+    nx, ny = 6, 4
+    row = [np.zeros((2, nx)), np.zeros((2, ny)), np.zeros((2, nx)), np.zeros((2, ny)),
+           np.ones(ny)]
+    pool, offs = pack_inlets([tuple(row)], nx, ny)
+    assert offs.shape == (1, 5)
+    for sd, bad in enumerate([np.zeros((2, nx + 1)), np.zeros((2, ny, 1)), np.zeros(nx),
+                              np.zeros((3, ny)), np.ones(ny + 1)]):
+        r = list(row)
+        r[sd] = bad
+        with pytest.raises(ValueError):
+            pack_inlets([tuple(r)], nx, ny)
+    with pytest.raises(ValueError):
+        pack_inlets([tuple(row[:4])], nx, ny)
+
+
+def test_lattice_walls_check_side_shapes():
+    """The per-call Zou-He methods refuse wall arrays of the wrong shape
+    before any kernel launch (the reference raises on a mis-shaped array)."""
+    from lbm_amd.lattice import lattice
+    from tests.cases import small_cavity
+    lat = lattice(small_cavity())
+    lat.u_left = np.zeros((2, lat.ny - 1))
+    with pytest.raises(ValueError, match="left"):
+        lat.zou_he_left_wall_velocity()
+    lat.u_top = np.zeros(lat.nx)
+    with pytest.raises(ValueError, match="top"):
+        lat.zou_he_top_wall_velocity()
+    lat.u_right = np.zeros((2, lat.ny))
+    lat.rho_right = np.zeros(lat.ny + 2)
+    with pytest.raises(ValueError, match="rho_right"):
+        lat.zou_he_right_wall_pressure()
