@@ -155,7 +155,8 @@ C4_PICKS = (23, 53, 180, 174, 3, 6)   # n = 65, 511, 499 (Re 993), 479 (tau 1.81
 def test_c4_ensemble_matches_oracle(storage):
     """The 256-cavity ensemble (n ~ U{64..512}, Re ~ logU[100, 1000], seed 0)
     in one batch, 400 iterations of the lid ramp; six cases against the
-    oracle: FP64 bit for bit, FP32 within 2e-5 of the velocity scale."""
+    oracle: FP64 bit for bit, FP32 within 2e-5 of the velocity scale plus
+    the FP32 rounding floor (1e-7)."""
     from lbm_amd.batch import cavity_amplitudes
     print(f"case: 256-cavity ensemble {storage}, {C4_IT} iterations", flush=True)
     cases = cavity_ensemble(n_cases=256, seed=0)
@@ -171,8 +172,14 @@ def test_c4_ensemble_matches_oracle(storage):
         if storage == "f64":
             _same(outs[c], ref, f"ensemble case {c}")
         else:
+            # 2e-5 of the velocity scale, plus the FP32 rounding floor of a
+            # density near 1 (one float ulp of 1.0 is 1.19e-7): at 400
+            # iterations the lid of the largest cavities has reached only
+            # 3e-4 (ramp sigma = 10 n), where that floor dominates
             uref = np.abs(ref["u"]).max()
-            assert np.abs(outs[c]["u"] - ref["u"]).max() <= 2e-5 * uref, c
+            err = np.abs(outs[c]["u"] - ref["u"]).max()
+            print(f"case {c}: max |u| {uref:.3e}, max u error {err:.3e}", flush=True)
+            assert err <= 2e-5 * uref + 1e-7, c
             assert np.abs(outs[c]["rho"] - ref["rho"]).max() <= 2e-6, c
 
 
