@@ -141,7 +141,7 @@ def run_form(app, lat, om_p, om_m, storage, steps, warmup, graph_len, cpt=0, vec
     u = torch.empty((2, nx, ny), dtype=torch.float64, device="cuda")
     dom.step(last, rho, u)
     dom.drag_only(last)
-    n_avg = 1000
+    n_avg = min(1000, last)
     drag = dom.drag_read(last - n_avg + 1, n_avg)
     g = torch.empty((9, nx, ny), dtype=torch.float64, device="cuda")
     gup = torch.empty_like(g)
@@ -585,6 +585,10 @@ def main():
                     help="16-bit forms: 4 nodes per work-item (4), one (1) or automatic (0)")
     ap.add_argument("--ny", type=int, default=2048)
     ap.add_argument("--cpu-iterations", type=int, default=160)
+    ap.add_argument("--accuracy-iterations", type=int, default=ACC_ITERATIONS,
+                    help="iterations of the obstacle-array run whose final state and drag "
+                         "are compared with FP64 (the configuration's 10 000; 0 = only the "
+                         "timed ones, for profiling runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sections", default="array,cavity,turek,batch,adjoint",
                     help="array (8192x2048 obstacle array, the headline), cavity (200x200), "
@@ -610,7 +614,7 @@ def main():
         print(f"# form {f}: {args.warmup} warmup + {args.steps} timed iterations",
               file=sys.stderr, flush=True)
         res[f] = run_form(app, lat, om_p, om_m, f, args.steps, args.warmup, args.graph_len,
-                          args.cpt, args.vector_width)
+                          args.cpt, args.vector_width, acc_iterations=args.accuracy_iterations)
         print(f"# form {f}: {res[f]['mlups']:.0f} MLUPS, {res[f]['ms'] / args.steps * 1e3:.1f} "
               f"us/iteration", file=sys.stderr, flush=True)
     ref = res.get("f64")

@@ -45,6 +45,10 @@ extern "C" {
                                equilibrium deviation at the node's velocity v,
                                which a tenth plane keeps as a half pair; float
                                arithmetic (fused and batched paths) */
+#define LBM_STORE_Q16   5   /* 16-bit fixed point of f_q - w_q on a per-direction
+                               affine map covering a caller-given range (the
+                               q16_range fields); float arithmetic (fused and
+                               batched paths) */
 
 /* Wall kinds of the fused boundary-condition plan. */
 #define LBM_BC_NONE      0
@@ -161,6 +165,9 @@ typedef struct {
                                    work-item with 8-byte accesses (taken when
                                    ny % 4 == 0 and nx*ny % (1024*cpt) == 0),
                                    0 or 1 = one node per work-item */
+    const double *q16_range_host; /* LBM_STORE_Q16: [9][2] (lo, hi) of h_q = f_q - w_q
+                                   per direction, -1 < lo < hi < 1; values outside
+                                   are clamped.  Ignored by the other forms. */
 } lbm_fused_desc;
 
 int  lbm_fused_create(const lbm_fused_desc *desc, lbm_fused **out);
@@ -175,6 +182,20 @@ void lbm_fused_destroy(lbm_fused *h);
  * written and LBM_ERR_ARG is returned. */
 int lbm_fused_set_inlets(lbm_fused *h, int row0, int nrows, const int32_t *offsets_host,
                          int pool_len, const double *pool_host, void *stream);
+
+/* Separable wall inputs (no counterpart in the reference, whose apps fill
+ * the wall arrays in Python every iteration; turek.py:94-109,
+ * cavity.py:65-76 are of this form): after lbm_fused_set_profiles, the
+ * value of side s (LBM_SIDE_*), element i, at iteration it is
+ * amp[it mod ring][s] * profile_s[i], and the inlet table of
+ * lbm_fused_set_inlets is no longer read.  profile_host[s] is u_bot/u_top
+ * [2][nx], u_left/u_right [2][ny] or rho_right [ny] (NULL = zeros);
+ * lbm_fused_set_amplitudes writes rows row0 .. row0+nrows-1 (mod ring) of
+ * amp_host[nrows][5].  Not for the cases of a batch (their amplitudes are
+ * the batch's). */
+int lbm_fused_set_profiles(lbm_fused *h, const double *const *profile_host, void *stream);
+int lbm_fused_set_amplitudes(lbm_fused *h, int64_t row0, int nrows, const double *amp_host,
+                             void *stream);
 
 /* Iteration 0: load g (double[9][nx][ny] on the device; NULL = the rest
  * state g = w), take moments, collide, store the post-collision field.  When rho_dev/u_dev are not
@@ -209,7 +230,9 @@ int lbm_drag_only(lbm_fused *h, int64_t it, void *stream);
  * Synchronises the stream. */
 int lbm_drag_read(lbm_fused *h, int64_t it0, int n, double *out_host, void *stream);
 
-/* Checkpoint of the post-collision state of iteration it (for rollback). */
+/* Checkpoint of the post-collision state of iteration it (for rollback).
+ * The handle keeps the two most recent checkpoints: a save replaces the
+ * older one (or the one of the same iteration); restore takes either. */
 int lbm_fused_save(lbm_fused *h, int64_t it, void *stream);
 int lbm_fused_restore(lbm_fused *h, int64_t it, void *stream);
 
@@ -220,6 +243,20 @@ int lbm_mass(lbm_fused *h, int64_t it, double *out_host, void *stream);
 
 /* Bytes of device memory held by the handle. */
 int64_t lbm_fused_device_bytes(const lbm_fused *h);
+
+/* ------------------------------------------------------------------------
+ * Observable histories (host code, no GPU; no counterpart in the reference's
+ * compiled code): buff.mv_avg (lbm/src/utils/buff.py:51-77), the triple
+ * moving average over the last fifth of a history, for k consecutive
+ * iterations at once, with the values np.sum gives (blocked pairwise
+ * summation).  The arrays are buff's: hist = the history, avg1..avg3 = the
+ * three averages, each starting with two zeros.  Iterations it0+1 .. it0+k:
+ * hist must hold entries 0 .. it0+k+1; entries it0+2 .. it0+k+1 of avg1..avg3
+ * are written, and obs_out[t] receives the value mv_avg returns at iteration
+ * it0+1+t.  Large chunks are split over up to 16 host threads.
+ * ---------------------------------------------------------------------- */
+int lbm_mv_avg_chunk(const double *hist, double *avg1, double *avg2, double *avg3,
+                     int64_t it0, int k, double *obs_out);
 
 /* ------------------------------------------------------------------------
  * Batched ensembles (no counterpart in the reference, which runs one case per
@@ -254,6 +291,7 @@ typedef struct {
     int32_t vector_width;           /* as lbm_fused_desc */
     int32_t ring;                   /* rows of the amplitude table and drag rings */
     int32_t grid;                   /* work-groups of the batched launch (0 = one per work item) */
+    const double *q16_range_host;   /* LBM_STORE_Q16: as lbm_fused_desc, common to all cases */
 } lbm_batch_desc;
 
 int  lbm_batch_create(const lbm_batch_desc *desc, lbm_batch **out);

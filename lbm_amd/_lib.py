@@ -14,8 +14,8 @@ LBM_ERR_ARG = -1
 LBM_ERR_HIP = -2
 LBM_ERR_UNSUPPORTED = -3
 
-STORE = {"f64": 0, "f32": 1, "f16": 2, "bf16": 3, "f16e": 4}
-STORE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "f16e": 2}
+STORE = {"f64": 0, "f32": 1, "f16": 2, "bf16": 3, "f16e": 4, "q16": 5}
+STORE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "f16e": 2, "q16": 2}
 BC_NONE, BC_VELOCITY, BC_PRESSURE = 0, 1, 2
 
 c_int = ctypes.c_int
@@ -41,6 +41,7 @@ class FusedDesc(ctypes.Structure):
         ("ring", c_i32),
         ("cells_per_thread", c_i32),
         ("vector_width", c_i32),
+        ("q16_range_host", c_dp),
     ]
 
 
@@ -64,6 +65,7 @@ class BatchDesc(ctypes.Structure):
     _fields_ = [
         ("n_cases", c_i32), ("cases", ctypes.POINTER(CaseDesc)), ("storage", c_i32),
         ("cells_per_thread", c_i32), ("vector_width", c_i32), ("ring", c_i32), ("grid", c_i32),
+        ("q16_range_host", c_dp),
     ]
 
 
@@ -97,6 +99,8 @@ _SIGNATURES = {
     "lbm_fused_create": ([ctypes.POINTER(FusedDesc), ctypes.POINTER(c_vp)], c_int),
     "lbm_fused_destroy": ([c_vp], None),
     "lbm_fused_set_inlets": ([c_vp, c_int, c_int, c_vp, c_int, c_vp, c_vp], c_int),
+    "lbm_fused_set_profiles": ([c_vp, c_vp, c_vp], c_int),
+    "lbm_fused_set_amplitudes": ([c_vp, c_i64, c_int, c_vp, c_vp], c_int),
     "lbm_init": ([c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "lbm_step": ([c_vp, c_i64, c_vp, c_vp, c_vp], c_int),
     "lbm_steps": ([c_vp, c_i64, c_int, c_vp], c_int),
@@ -108,6 +112,7 @@ _SIGNATURES = {
     "lbm_fused_restore": ([c_vp, c_i64, c_vp], c_int),
     "lbm_mass": ([c_vp, c_i64, c_vp, c_vp], c_int),
     "lbm_fused_device_bytes": ([c_vp], c_i64),
+    "lbm_mv_avg_chunk": ([c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_vp], c_int),
     "lbm_batch_create": ([ctypes.POINTER(BatchDesc), ctypes.POINTER(c_vp)], c_int),
     "lbm_batch_destroy": ([c_vp], None),
     "lbm_batch_case": ([c_vp, c_int, ctypes.POINTER(c_vp)], c_int),

@@ -33,6 +33,17 @@ class channel_app(base_app):
         val = it
         return (1.0 - math.exp(-val ** 2 / (2.0 * self.sigma ** 2)))
 
+    def separable_inlets(self, lattice):
+        """set_inlets(it) as amplitude(it) x profile (the fused run's device
+        tables): u_left = (ramp(it) u_lbm) x the parabola, rho_right =
+        1 x rho_lbm, the other sides zero."""
+        prof = self._inlet_profile(lattice)
+        rho = np.full(lattice.ny, float(self.rho_lbm))
+
+        def amp(it):
+            return (0.0, self.ramp(it) * self.u_lbm, 0.0, 0.0, 1.0)
+        return [None, prof, None, None, rho], amp
+
     def set_inlets(self, lattice, it):
         # Same values as the reference's per-node loop: (ret*u_lbm) * profile_j.
         ret = self.ramp(it)

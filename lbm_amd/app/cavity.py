@@ -44,6 +44,18 @@ class cavity(base_app):
         lattice.equilibrium()
         lattice.g = lattice.g_eq.copy()
 
+    def separable_inlets(self, lattice):
+        """set_inlets(it) as amplitude(it) x profile: u_top = (u_lbm ret) x (1, 0)
+        per node, the other sides zero."""
+        top = np.zeros((2, lattice.nx))
+        top[0, :] = 1.0
+
+        def amp(it):
+            val = it
+            ret = (1.0 - math.exp(-val ** 2 / (2.0 * self.sigma ** 2)))
+            return (0.0, 0.0, self.u_lbm * ret, 0.0, 0.0)
+        return [None, None, top, None, None], amp
+
     def set_inlets(self, lattice, it):
         val = it
         ret = (1.0 - math.exp(-val ** 2 / (2.0 * self.sigma ** 2)))

@@ -67,6 +67,48 @@ class turek(channel_app):
         self._dl_file.write('{} {} {} {} {} {} {}\n'.format(it * self.dt, Cx, Cy, avg_Cx,
                                                             avg_Cy, dcx, dcy))
 
+    def replay_chunk(self, lattice, it0, it1, at):
+        """The hooks of iterations it0 .. it1-1 in the reference's order
+        (printings, outputs, observables, check_stop; run.py:24-54) with the
+        observables vectorised: drag and lift of every iteration from the
+        device sums, the moving averages by buff.add_chunk (the values mv_avg
+        gives), the drag_lift lines in one write.  Returns the iteration
+        check_stop ends the run at, or None."""
+        from lbm_amd.app.base_app import base_app
+        cls = type(self)
+        if (cls.observables is not turek.observables or cls.add_buff is not turek.add_buff or
+                cls.check_stop is not base_app.check_stop):
+            return NotImplemented          # hooks of a subclass: replay them per iteration
+        cx, cy = lattice.drag_lift_chunk(self.obstacles[0], self.rho_lbm, self.u_avg, self.D_lbm)
+        db, lb = self.drag_buff, self.lift_buff
+        prev = (db.obs, lb.obs)
+        od, gd = db.add_chunk(cx)
+        ol, gl = lb.add_chunk(cy)
+        n = it1 - it0
+        stop = None
+        if self.stop == 'it' and it1 - 1 >= self.it_max:
+            stop = max(self.it_max, it0)
+        elif self.stop == 'obs' and db.obs_cv and lb.obs_cv:
+            # the buffs count the observables calls, 1 at iteration 0
+            stop = max(max(db.obs_cv_it, lb.obs_cv_it) - 1, it0)
+        m = n if stop is None else stop - it0 + 1
+        db.truncate_chunk(m)
+        lb.truncate_chunk(m)
+        for t in range(m):
+            at(it0 + t)
+            db.obs, lb.obs = prev if t == 0 else (od[t - 1], ol[t - 1])
+            self.printings(it0 + t)
+            self.outputs(lattice, it0 + t)
+        db.obs, lb.obs = od[m - 1], ol[m - 1]
+        dt = self.dt
+        self._dl_file.write(''.join(
+            '{} {} {} {} {} {} {}\n'.format((it0 + t) * dt, float(cx[t]), float(cy[t]),
+                                            float(od[t]), float(ol[t]), float(gd[t]),
+                                            float(gl[t])) for t in range(m)))
+        if stop is not None:
+            self.check_stop(stop)
+        return stop
+
     def finalize(self, lattice):
         f = getattr(self, "_dl_file", None)
         if f is not None:

@@ -20,7 +20,7 @@ from lbm_amd.domain import FusedDomain
 
 class Batch:
     def __init__(self, cases, storage="f32", ring=1002, cells_per_thread=0, vector_width=0,
-                 grid=0):
+                 grid=0, q16_range=None):
         self.L = _lib.load()
         self.cases = cases
         self.storage = storage
@@ -58,6 +58,10 @@ class Batch:
         bd.vector_width = int(vector_width)
         bd.ring = self.ring
         bd.grid = int(grid)
+        if q16_range is not None:
+            q = np.ascontiguousarray(q16_range, dtype=np.float64).reshape(9, 2)
+            self._keep.append(q)
+            bd.q16_range_host = q.ctypes.data_as(_lib.c_dp)
         h = _lib.c_vp()
         _lib.check(self.L.lbm_batch_create(ctypes.byref(bd), ctypes.byref(h)), "lbm_batch_create")
         self.h = h

@@ -17,9 +17,16 @@ class _Grow:
         self._a = np.zeros(1024)
         self.n = 2
 
+    def reserve(self, extra):
+        if self.n + extra > len(self._a):
+            cap = len(self._a)
+            while cap < self.n + extra:
+                cap *= 2
+            self._a = np.concatenate([self._a, np.zeros(cap - len(self._a))])
+
     def append(self, v):
         if self.n == len(self._a):
-            self._a = np.concatenate([self._a, np.zeros(len(self._a))])
+            self.reserve(1)
         self._a[self.n] = v
         self.n += 1
 
@@ -44,6 +51,7 @@ class buff:
         self.obs_cv_nb = obs_cv_nb
         self.obs_cv_cnt = 0
         self.obs_cv = False
+        self.obs_cv_it = None     # the value of `it` when obs_cv was set
 
     @property
     def buff(self):
@@ -72,6 +80,75 @@ class buff:
     def p_avg(self, b, i, j):
         return np.sum(b[i:j]) / (float(j - i + 1))
 
+    def add_chunk(self, values):
+        """``add(v); mv_avg()`` for each v of ``values`` in order, with the
+        same results (the window sums run in the D2Q9 library's host code,
+        lbm_mv_avg_chunk, bit for bit what np.sum gives).  Returns the arrays
+        of what mv_avg returned: (obs, growth)."""
+        from lbm_amd import _lib
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        k = len(v)
+        obs = np.zeros(k)
+        growth = np.zeros(k)
+        if k == 0:
+            return obs, growth
+        for g in (self._buff, self._avg1, self._avg2, self._avg3):
+            g.reserve(k)
+        it0 = self.it
+        n0 = self._buff.n
+        self._buff._a[n0:n0 + k] = v
+        self._buff.n += k
+        self.it += k
+        ptr = lambda g: g._a.ctypes.data_as(_lib.c_vp)
+        _lib.check(_lib.load().lbm_mv_avg_chunk(ptr(self._buff), ptr(self._avg1), ptr(self._avg2),
+                                                ptr(self._avg3), it0, k,
+                                                obs.ctypes.data_as(_lib.c_vp)),
+                   "lbm_mv_avg_chunk")
+        for g in (self._avg1, self._avg2, self._avg3):
+            g.n += k
+        a3 = self._avg3._a
+        self._chunk = dict(k=k, obs=(self.obs, self.obs_cv_cnt, self.obs_cv, self.obs_cv_it),
+                           cnt=np.zeros(k, dtype=np.int64), obs_vals=obs)
+        for t in range(k):
+            it = it0 + 1 + t
+            if it <= 10:
+                continue
+            it_s = math.floor(it * 4 / 5)
+            db = a3[it] - a3[it_s]
+            gr = db / ((it - it_s + 1) * self.dt)
+            growth[t] = gr
+            if abs(gr) < self.obs_cv_ct:
+                self.obs_cv_cnt += 1
+            else:
+                self.obs_cv_cnt = 0
+            if self.obs_cv_cnt > self.obs_cv_nb and not self.obs_cv:
+                self.obs_cv = True
+                self.obs_cv_it = it
+            self._chunk["cnt"][t] = self.obs_cv_cnt
+        self.obs = obs[-1]
+        return obs, growth
+
+    def truncate_chunk(self, m):
+        """Keep only the first m values of the last add_chunk call: the
+        history, averages, counters and flag as after those m iterations."""
+        c = self._chunk
+        drop = c["k"] - m
+        if drop <= 0:
+            return
+        for g in (self._buff, self._avg1, self._avg2, self._avg3):
+            g.n -= drop
+            g._a[g.n:g.n + drop] = 0.0
+        self.it -= drop
+        obs0, cnt0, cv0, cvit0 = c["obs"]
+        if m == 0:
+            self.obs, self.obs_cv_cnt, self.obs_cv, self.obs_cv_it = obs0, cnt0, cv0, cvit0
+        else:
+            self.obs = c["obs_vals"][m - 1]
+            self.obs_cv_cnt = int(c["cnt"][m - 1])
+            if self.obs_cv and not cv0 and self.obs_cv_it > self.it:
+                self.obs_cv, self.obs_cv_it = False, None
+        c["k"] = m
+
     def mv_avg(self):
         it_s = math.floor(self.it * 4 / 5)
         it_e = self.it
@@ -91,6 +168,7 @@ class buff:
                 self.obs_cv_cnt += 1
             else:
                 self.obs_cv_cnt = 0
-            if self.obs_cv_cnt > self.obs_cv_nb:
+            if self.obs_cv_cnt > self.obs_cv_nb and not self.obs_cv:
                 self.obs_cv = True
+                self.obs_cv_it = self.it
         return self.obs, growth

@@ -23,9 +23,11 @@
 #include "../../include/lbm_d2q9.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -38,137 +40,6 @@ int set_error(int code, const std::string &msg)
     g_last_error = msg;
     return code;
 }
-
-// ---------------------------------------------------------------- storage --
-// A state buffer holds nine population planes of N nodes (plane q, node
-// x*ny + y).  ld(p, N, q, node) returns the population f_q; the float forms
-// store a deviation and ldh returns h_q = f_q - w_q.
-//   F64             f_q in double
-//   F32, F16, BF16  h_q in float, IEEE half or bfloat16
-//   F16E            g_q = h_q - E_q(v) in IEEE half, with
-//                   E_q(v) = w_q (3 c_q.v + 4.5 (c_q.v)^2 - 1.5 v.v)
-//                   the equilibrium deviation at the node's velocity v rounded
-//                   to half precision; a tenth plane holds v as packed (vx, vy)
-//                   half pairs.  g_q is the density and non-equilibrium part
-//                   of h_q (~1e-3 against ~2e-2 for h_q in a u ~ 0.05 flow), so
-//                   half precision resolves it 10-30 times finer.  Writer and
-//                   reader evaluate E_q from the same half-precision v with the
-//                   same operations, so the shift cancels exactly.
-template <int S> struct Store;
-
-// Float -> half conversions take their operand through an empty asm, so the
-// compiler cannot fold the preceding multiply-add into a mixed-precision FMA
-// that rounds to half once (v_fma_mixlo_f16): every kernel then rounds twice,
-// to float and to half, and all kernel shapes store identical values.
-__device__ __forceinline__ _Float16 to_half(float v)
-{
-    asm("" : "+v"(v));
-    return (_Float16)v;
-}
-
-// F16E: offset (in halves) of the velocity plane, 256-byte aligned
-__host__ __device__ constexpr size_t vel_offset(int N) { return ((size_t)9 * N + 127) / 128 * 128; }
-
-template <> struct Store<LBM_STORE_F64> {
-    using s_t = double;
-    using T = double;
-    static __device__ __forceinline__ T ld(const s_t *p, int N, int q, size_t c)
-    {
-        return p[(size_t)q * N + c];
-    }
-    static __device__ __forceinline__ void st(s_t *p, size_t i, int, T v) { p[i] = v; }
-};
-
-template <> struct Store<LBM_STORE_F32> {
-    using s_t = float;
-    using T = float;
-    static __device__ __forceinline__ float ldh(const s_t *p, int N, int q, size_t c)
-    {
-        return p[(size_t)q * N + c];
-    }
-    static __device__ __forceinline__ void sth(s_t *p, size_t i, float h) { p[i] = h; }
-    static __device__ __forceinline__ T ld(const s_t *p, int N, int q, size_t c)
-    {
-        return weight<float>(q) + ldh(p, N, q, c);
-    }
-};
-
-template <> struct Store<LBM_STORE_F16> {
-    using s_t = _Float16;
-    using T = float;
-    static __device__ __forceinline__ float ldh(const s_t *p, int N, int q, size_t c)
-    {
-        return (float)p[(size_t)q * N + c];
-    }
-    static __device__ __forceinline__ void sth(s_t *p, size_t i, float h) { p[i] = to_half(h); }
-    static __device__ __forceinline__ T ld(const s_t *p, int N, int q, size_t c)
-    {
-        return weight<float>(q) + ldh(p, N, q, c);
-    }
-};
-
-__device__ __forceinline__ uint16_t f32_to_bf16(float v)
-{
-    uint32_t u = __float_as_uint(v);
-    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-    return (uint16_t)(u >> 16);
-}
-
-template <> struct Store<LBM_STORE_BF16> {
-    using s_t = uint16_t;
-    using T = float;
-    static __device__ __forceinline__ float ldh(const s_t *p, int N, int q, size_t c)
-    {
-        return __uint_as_float((uint32_t)p[(size_t)q * N + c] << 16);
-    }
-    static __device__ __forceinline__ void sth(s_t *p, size_t i, float h) { p[i] = f32_to_bf16(h); }
-    static __device__ __forceinline__ T ld(const s_t *p, int N, int q, size_t c)
-    {
-        return weight<float>(q) + ldh(p, N, q, c);
-    }
-};
-
-// F16E: E_q(v), explicit fmaf only, so every caller rounds identically
-// c_q.v is taken as +-vx, +-vy or their sum (no multiplications by the
-// integer components, which the compiler may not drop without fast-math), and
-// E_q = cu (3 w_q + 4.5 w_q cu) - 1.5 w_q v.v with the products of the weights
-// folded into constants (three operations after v.v and cu); q may be a
-// run-time value (the link path), the selection then yields the same
-// operations.  Writer and reader evaluate it on the same half-precision
-// velocity, so the shift cancels exactly whatever its own rounding.
-__device__ __forceinline__ float eq_shift(int q, float vx, float vy)
-{
-    const float vv = __builtin_fmaf(vy, vy, vx * vx);
-    const float ax = CX[q] > 0 ? vx : -vx, ay = CY[q] > 0 ? vy : -vy;
-    const float cu = CY[q] == 0 ? (CX[q] == 0 ? 0.0f : ax) : (CX[q] == 0 ? ay : ax + ay);
-    const float w = weight<float>(q);
-    return __builtin_fmaf(cu, __builtin_fmaf(4.5f * w, cu, 3.0f * w), (-1.5f * w) * vv);
-}
-
-__device__ __forceinline__ void vel_unpack(uint32_t v, float &vx, float &vy)
-{
-    vx = (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu));
-    vy = (float)__builtin_bit_cast(_Float16, (uint16_t)(v >> 16));
-}
-
-template <> struct Store<LBM_STORE_F16E> {
-    using s_t = _Float16;
-    using T = float;
-    static __device__ __forceinline__ const uint32_t *vel(const s_t *p, int N)
-    {
-        return reinterpret_cast<const uint32_t *>(p + vel_offset(N));
-    }
-    static __device__ __forceinline__ float ldh(const s_t *p, int N, int q, size_t c)
-    {
-        float vx, vy;
-        vel_unpack(vel(p, N)[c], vx, vy);
-        return eq_shift(q, vx, vy) + (float)p[(size_t)q * N + c];
-    }
-    static __device__ __forceinline__ T ld(const s_t *p, int N, int q, size_t c)
-    {
-        return weight<float>(q) + ldh(p, N, q, c);
-    }
-};
 
 // ------------------------------------------------------------------ links --
 // One obstacle link (fluid node `cell`, direction q towards the solid), with
@@ -214,6 +85,218 @@ struct Params {
     int n_link_cells;
     int tile_shift;        // log2(tile_cells)
     int rev;               // sweep the XCD slabs last tile first (odd iterations)
+    // Q16: h_q = qo[q] + qs[q] * i (decode); i = rint(h_q * qi[q] + qc[q]),
+    // clamped to [-32767, 32767] (encode); qi = 1/qs, qc = -qo/qs.  qtab holds
+    // the same four rows in device memory for directions known only at run
+    // time (obstacle links): indexing these arrays with a run-time value would
+    // move the whole parameter block to scratch memory.
+    float qs[9], qo[9], qi[9], qc[9];
+    const float *qtab;     // [4][9]: qs, qo, qi, qc
+};
+
+// ---------------------------------------------------------------- storage --
+// A state buffer holds nine population planes of N nodes (plane q, node
+// x*ny + y).  ld(p, buf, q, node) returns the population f_q; the float forms
+// store a deviation and ldh returns h_q = f_q - w_q; sth stores h_q.
+//   F64             f_q in double
+//   F32, F16, BF16  h_q in float, IEEE half or bfloat16
+//   F16E            g_q = h_q - E_q(v) in IEEE half, with
+//                   E_q(v) = w_q (3 c_q.v + 4.5 (c_q.v)^2 - 1.5 v.v)
+//                   the equilibrium deviation at the node's velocity v rounded
+//                   to half precision; a tenth plane holds v as packed (vx, vy)
+//                   half pairs.  g_q is the density and non-equilibrium part
+//                   of h_q (~1e-3 against ~2e-2 for h_q in a u ~ 0.05 flow), so
+//                   half precision resolves it 10-30 times finer.  Writer and
+//                   reader evaluate E_q from the same half-precision v with the
+//                   same operations, so the shift cancels exactly.
+//   Q16             h_q in 16-bit fixed point on a per-direction affine map,
+//                   h_q = qo[q] + qs[q] i with i in [-32767, 32767]: the 65535
+//                   levels cover the range of h_q the flow can reach (set by
+//                   the caller from its velocity bound), uniformly, where IEEE
+//                   half spends most of its codes near zero.  Values outside
+//                   the range are clamped to it.
+template <int S> struct Store;
+
+__host__ __device__ constexpr bool is16(int S)
+{
+    return S == LBM_STORE_F16 || S == LBM_STORE_BF16 || S == LBM_STORE_F16E ||
+           S == LBM_STORE_Q16;
+}
+
+// Float -> half conversions take their operand through an empty asm, so the
+// compiler cannot fold the preceding multiply-add into a mixed-precision FMA
+// that rounds to half once (v_fma_mixlo_f16): every kernel then rounds twice,
+// to float and to half, and all kernel shapes store identical values.
+__device__ __forceinline__ _Float16 to_half(float v)
+{
+    asm("" : "+v"(v));
+    return (_Float16)v;
+}
+
+// F16E: offset (in halves) of the velocity plane, 256-byte aligned
+__host__ __device__ constexpr size_t vel_offset(int N) { return ((size_t)9 * N + 127) / 128 * 128; }
+
+template <> struct Store<LBM_STORE_F64> {
+    using s_t = double;
+    using T = double;
+    static __device__ __forceinline__ T ld(const Params &p, const s_t *b, int q, size_t c)
+    {
+        return b[(size_t)q * p.N + c];
+    }
+    static __device__ __forceinline__ void st(const Params &, s_t *b, int, size_t i, T v)
+    {
+        b[i] = v;
+    }
+};
+
+template <> struct Store<LBM_STORE_F32> {
+    using s_t = float;
+    using T = float;
+    static __device__ __forceinline__ float ldh(const Params &p, const s_t *b, int q, size_t c)
+    {
+        return b[(size_t)q * p.N + c];
+    }
+    static __device__ __forceinline__ void sth(const Params &, s_t *b, int, size_t i, float h)
+    {
+        b[i] = h;
+    }
+    static __device__ __forceinline__ T ld(const Params &p, const s_t *b, int q, size_t c)
+    {
+        return weight<float>(q) + ldh(p, b, q, c);
+    }
+};
+
+template <> struct Store<LBM_STORE_F16> {
+    using s_t = _Float16;
+    using T = float;
+    static __device__ __forceinline__ float ldh(const Params &p, const s_t *b, int q, size_t c)
+    {
+        return (float)b[(size_t)q * p.N + c];
+    }
+    static __device__ __forceinline__ void sth(const Params &, s_t *b, int, size_t i, float h)
+    {
+        b[i] = to_half(h);
+    }
+    static __device__ __forceinline__ T ld(const Params &p, const s_t *b, int q, size_t c)
+    {
+        return weight<float>(q) + ldh(p, b, q, c);
+    }
+};
+
+__device__ __forceinline__ uint16_t f32_to_bf16(float v)
+{
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+template <> struct Store<LBM_STORE_BF16> {
+    using s_t = uint16_t;
+    using T = float;
+    static __device__ __forceinline__ float ldh(const Params &p, const s_t *b, int q, size_t c)
+    {
+        return __uint_as_float((uint32_t)b[(size_t)q * p.N + c] << 16);
+    }
+    static __device__ __forceinline__ void sth(const Params &, s_t *b, int, size_t i, float h)
+    {
+        b[i] = f32_to_bf16(h);
+    }
+    static __device__ __forceinline__ T ld(const Params &p, const s_t *b, int q, size_t c)
+    {
+        return weight<float>(q) + ldh(p, b, q, c);
+    }
+};
+
+// Q16 encode: x = h qi + qc with qc = 1.5 * 2^23 - m (one FMA): the sum lies
+// in [2^23, 2^24), where floats are the integers, so x is rounded to
+// 1.5 * 2^23 + rint(h / qs) - m, whose low 16 bits are the code i in two's
+// complement; x is clamped to the codes -32767 .. 32767.  A non-finite h
+// gets the code -32768, which the density sum turns back into a NaN.  The
+// multiply-add goes through an empty asm so that every kernel shape rounds
+// it the same way.
+constexpr float Q16_MAGIC = 12582912.0f;   // 1.5 * 2^23
+__device__ __forceinline__ uint16_t q16_encode(float h, float qi, float qc)
+{
+    float x = __builtin_fmaf(h, qi, qc);
+    asm("" : "+v"(x));
+    const float c = __builtin_fminf(__builtin_fmaxf(x, Q16_MAGIC - 32767.0f), Q16_MAGIC + 32767.0f);
+    return x == x ? (uint16_t)(__float_as_uint(c) & 0xffffu) : (uint16_t)0x8000u;
+}
+
+__device__ __forceinline__ float q16_decode(int16_t i, float qs, float qo)
+{
+    return __builtin_fmaf(qs, (float)i, qo);
+}
+
+// Affine-map coefficient `row` (0 qs, 1 qo, 2 qi, 3 qc) of direction q: from
+// the kernel arguments when q is a compile-time constant after inlining (the
+// unrolled node loops), else from the device table.
+__device__ __forceinline__ float q16_coef(const Params &p, int row, int q)
+{
+    if (__builtin_constant_p(q)) {
+        const float *r = row == 0 ? p.qs : row == 1 ? p.qo : row == 2 ? p.qi : p.qc;
+        return r[q];
+    }
+    return p.qtab[row * 9 + q];
+}
+
+template <> struct Store<LBM_STORE_Q16> {
+    using s_t = int16_t;
+    using T = float;
+    static __device__ __forceinline__ float ldh(const Params &p, const s_t *b, int q, size_t c)
+    {
+        return q16_decode(b[(size_t)q * p.N + c], q16_coef(p, 0, q), q16_coef(p, 1, q));
+    }
+    static __device__ __forceinline__ void sth(const Params &p, s_t *b, int q, size_t i, float h)
+    {
+        b[i] = (int16_t)q16_encode(h, q16_coef(p, 2, q), q16_coef(p, 3, q));
+    }
+    static __device__ __forceinline__ T ld(const Params &p, const s_t *b, int q, size_t c)
+    {
+        return weight<float>(q) + ldh(p, b, q, c);
+    }
+};
+
+// F16E: E_q(v), explicit fmaf only, so every caller rounds identically
+// c_q.v is taken as +-vx, +-vy or their sum (no multiplications by the
+// integer components, which the compiler may not drop without fast-math), and
+// E_q = cu (3 w_q + 4.5 w_q cu) - 1.5 w_q v.v with the products of the weights
+// folded into constants (three operations after v.v and cu); q may be a
+// run-time value (the link path), the selection then yields the same
+// operations.  Writer and reader evaluate it on the same half-precision
+// velocity, so the shift cancels exactly whatever its own rounding.
+__device__ __forceinline__ float eq_shift(int q, float vx, float vy)
+{
+    const float vv = __builtin_fmaf(vy, vy, vx * vx);
+    const float ax = CX[q] > 0 ? vx : -vx, ay = CY[q] > 0 ? vy : -vy;
+    const float cu = CY[q] == 0 ? (CX[q] == 0 ? 0.0f : ax) : (CX[q] == 0 ? ay : ax + ay);
+    const float w = weight<float>(q);
+    return __builtin_fmaf(cu, __builtin_fmaf(4.5f * w, cu, 3.0f * w), (-1.5f * w) * vv);
+}
+
+__device__ __forceinline__ void vel_unpack(uint32_t v, float &vx, float &vy)
+{
+    vx = (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu));
+    vy = (float)__builtin_bit_cast(_Float16, (uint16_t)(v >> 16));
+}
+
+template <> struct Store<LBM_STORE_F16E> {
+    using s_t = _Float16;
+    using T = float;
+    static __device__ __forceinline__ const uint32_t *vel(const s_t *b, int N)
+    {
+        return reinterpret_cast<const uint32_t *>(b + vel_offset(N));
+    }
+    static __device__ __forceinline__ float ldh(const Params &p, const s_t *b, int q, size_t c)
+    {
+        float vx, vy;
+        vel_unpack(vel(b, p.N)[c], vx, vy);
+        return eq_shift(q, vx, vy) + (float)b[(size_t)q * p.N + c];
+    }
+    static __device__ __forceinline__ T ld(const Params &p, const s_t *b, int q, size_t c)
+    {
+        return weight<float>(q) + ldh(p, b, q, c);
+    }
 };
 
 enum { MODE_INIT = 0, MODE_STEP = 1, MODE_EPILOGUE = 2, MODE_DRAG = 3 };
@@ -254,7 +337,7 @@ __device__ __forceinline__ void pull_checked(const Params &p, int x, int y,
     for (int q = 0; q < 9; ++q) {
         const int sx = x - CX[q], sy = y - CY[q];
         f[q] = (sx >= 0 && sx < p.nx && sy >= 0 && sy < p.ny)
-                   ? St::ld(src, p.N, q, (size_t)sx * p.ny + sy)
+                   ? St::ld(p, src, q, (size_t)sx * p.ny + sy)
                    : T(0);
     }
 }
@@ -269,10 +352,10 @@ __device__ __forceinline__ float link_h(const Params &p, int cell, const LinkRec
 {
     using St = Store<S>;
     const auto *src = static_cast<const typename St::s_t *>(p.src);
-    const float A = St::ldh(src, p.N, L.q, cell);
+    const float A = St::ldh(p, src, L.q, cell);
     if (!L.mode) return A;
-    const float B = St::ldh(src, p.N, L.planeB, L.cellB);
-    const float C = St::ldh(src, p.N, L.planeC, L.cellC);
+    const float B = St::ldh(p, src, L.planeB, L.cellB);
+    const float C = St::ldh(p, src, L.planeC, L.cellC);
     return ((float)L.c0 * A + (float)L.c1 * B) + (float)L.c2 * C;
 }
 
@@ -289,11 +372,11 @@ __device__ __forceinline__ void apply_links(const Params &p, int cell, int lo, i
         const LinkRec L = p.links[k];
         T v;
         if constexpr (S == LBM_STORE_F64) {
-            const T A = St::ld(src, p.N, L.q, cell);
+            const T A = St::ld(p, src, L.q, cell);
             v = A;
             if (L.mode) {
-                const T B = St::ld(src, p.N, L.planeB, L.cellB);
-                const T C = St::ld(src, p.N, L.planeC, L.cellC);
+                const T B = St::ld(p, src, L.planeB, L.cellB);
+                const T C = St::ld(p, src, L.planeC, L.cellC);
                 v = (T(L.c0) * A + T(L.c1) * B) + T(L.c2) * C;
             }
         } else {
@@ -410,14 +493,14 @@ __device__ __forceinline__ void store_drag_range(const Params &p, int cell, int 
 #pragma unroll
             for (int q = 1; q < 9; ++q)
                 if (q == L.qb) fb = f[q];
-            v = St::ld(src, p.N, L.q, cell) + fb;
+            v = St::ld(p, src, L.q, cell) + fb;
         } else {
             // f[qb] = fl(w + h) with |h| < w/2, so f[qb] - w is exact (Sterbenz)
             float hb = 0.0f;
 #pragma unroll
             for (int q = 1; q < 9; ++q)
                 if (q == L.qb) hb = f[q] - weight<float>(q);
-            const double hA = (double)St::ldh(src, p.N, L.q, cell);
+            const double hA = (double)St::ldh(p, src, L.q, cell);
             v = 2.0 * weight<double>(L.q) + (hA + (double)hb);
         }
         p.drag_ring[(size_t)row * p.n_links + L.dslot] = v;
@@ -437,7 +520,7 @@ __device__ __forceinline__ void store_drag_range_h(const Params &p, int cell, in
 #pragma unroll
         for (int q = 1; q < 9; ++q)
             if (q == L.qb) hb = h[q];
-        const double hA = (double)St::ldh(src, p.N, L.q, cell);
+        const double hA = (double)St::ldh(p, src, L.q, cell);
         p.drag_ring[(size_t)row * p.n_links + L.dslot] =
             2.0 * weight<double>(L.q) + (hA + (double)hb);
     }
@@ -479,12 +562,12 @@ __device__ __forceinline__ void collide_store(const Params &p, int cell,
         const T t = T(3.0) * (ux * T(CX[q]) + uy * T(CY[q]));
         e[q] = (T(1.0) + t + T(0.5) * (t * t) - v) * (r * weight<T>(q));
     }
-    St::st(dst, cell, 0, T(p.b0) * f[0] + T(p.om_p) * e[0]);
+    St::st(p, dst, 0, cell, T(p.b0) * f[0] + T(p.om_p) * e[0]);
     const T a1 = T(p.a1), a2 = T(p.a2), a3 = T(p.a3), a4 = T(p.a4);
 #pragma unroll
     for (int q = 1; q < 9; ++q) {
         const int qb = NS[q];
-        St::st(dst, (size_t)q * p.N + cell, q, a1 * f[q] - a2 * f[qb] + a3 * e[q] + a4 * e[qb]);
+        St::st(p, dst, q, (size_t)q * p.N + cell, a1 * f[q] - a2 * f[qb] + a3 * e[q] + a4 * e[qb]);
     }
 }
 
@@ -569,7 +652,7 @@ __device__ __forceinline__ void collide_store_h(const Params &p, int cell, const
             dst[(size_t)q * p.N + cell] = to_half(h[q] - eq_shift(q, vx, vy));
     } else {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) St::sth(dst, (size_t)q * p.N + cell, h[q]);
+        for (int q = 0; q < 9; ++q) St::sth(p, dst, q, (size_t)q * p.N + cell, h[q]);
     }
 }
 
@@ -639,10 +722,10 @@ __global__ __launch_bounds__(256) void k_fused(Params p)
                 // the float forms' stored populations are w_q + h_q with the
                 // exact double weight w_q
                 if constexpr (S == LBM_STORE_F64)
-                    p.gup_out[(size_t)q * p.N + cell] = St::ld(src, p.N, q, cell);
+                    p.gup_out[(size_t)q * p.N + cell] = St::ld(p, src, q, cell);
                 else
                     p.gup_out[(size_t)q * p.N + cell] =
-                        weight<double>(q) + (double)St::ldh(src, p.N, q, cell);
+                        weight<double>(q) + (double)St::ldh(p, src, q, cell);
             }
             if (!interior && p.rho_out) {
                 p.rho_out[cell] = (double)r;
@@ -718,7 +801,7 @@ __device__ __forceinline__ void wall_routine(const Params &p, int cell, int row,
 {
     using T = typename Store<S>::T;
     const int x = node_x(p, cell);
-    if constexpr (S == LBM_STORE_F16 || S == LBM_STORE_BF16 || S == LBM_STORE_F16E) {
+    if constexpr (is16(S)) {
         // wall velocities are read-only during the launch: scalar loads
         using cd = const __attribute__((address_space(4))) double;
         using ci = const __attribute__((address_space(4))) int;
@@ -809,7 +892,7 @@ __device__ __forceinline__ void link_pass(const Params &p, int r0, int r1, int f
             atomicOr(&lf[o], 1u << L.qb);
             const int x = node_x(p, L.cell), y = L.cell - x * p.ny;
             if ((unsigned)(x - 1) < (unsigned)(lx - 1) && (unsigned)(y - 1) < (unsigned)(ly - 1)) {
-                const double hA = (double)St::ldh(src, p.N, L.q, L.cell);
+                const double hA = (double)St::ldh(p, src, L.q, L.cell);
                 p.drag_ring[(size_t)row * p.n_links + L.dslot] =
                     2.0 * weight<double>(L.q) + (hA + (double)v);
             }
@@ -868,9 +951,9 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
             int sc = lc - (CX[q] * p.ny + CY[q]);
             if (wall[c] && ((CY[q] == 1 && y == 0) || (CY[q] == -1 && y == ly))) sc = lc;
             if constexpr (S == LBM_STORE_F64)
-                v[c][q] = St::ld(src, p.N, q, (size_t)sc);
+                v[c][q] = St::ld(p, src, q, (size_t)sc);
             else
-                v[c][q] = St::ldh(src, p.N, q, (size_t)sc);
+                v[c][q] = St::ldh(p, src, q, (size_t)sc);
         }
     }
 #pragma unroll
@@ -1168,10 +1251,11 @@ __global__ void k_drag_reduce(const double *__restrict__ ringbuf, int n_links, i
 // w_q + h_q with the exact double weights w_q), and k_sum_serial adds
 // N * sum_q w_q once.  Work-group partials land in part[blockIdx.x].
 template <int S>
-__global__ __launch_bounds__(256) void k_mass_partial(const void *buf, int N, double *part)
+__global__ __launch_bounds__(256) void k_mass_partial(Params p, double *part)
 {
     using St = Store<S>;
-    const auto *src = static_cast<const typename St::s_t *>(buf);
+    const auto *src = static_cast<const typename St::s_t *>(p.src);
+    const int N = p.N;
     __shared__ double sh[256];
     double acc = 0.0;
     for (int c = blockIdx.x * 256 + threadIdx.x; c < N; c += gridDim.x * 256) {
@@ -1179,9 +1263,11 @@ __global__ __launch_bounds__(256) void k_mass_partial(const void *buf, int N, do
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
             if constexpr (S == LBM_STORE_F64)
-                s += St::ld(src, N, q, c);
+                s += St::ld(p, src, q, c);
             else
-                s += (double)St::ldh(src, N, q, c);
+                s += (double)St::ldh(p, src, q, c);
+            if constexpr (S == LBM_STORE_Q16)   // the code of a non-finite value
+                if (src[(size_t)q * N + c] == (int16_t)-32768) s = __builtin_nan("");
         }
         acc += s;
     }
@@ -1218,6 +1304,28 @@ size_t state_bytes(int s, int N)
     return store_bytes(s) * 9 * (size_t)N;
 }
 
+// Q16 affine maps from the range [lo, hi] of h_q per direction: the codes
+// -32767..32767 span it, h = qs i + qo.  qs is (hi - lo) / 65534 rounded up
+// to 8 significant bits and qo = qs m with m the integer nearest the range's
+// centre over qs, so qs m is exact in float: h = 0 (the rest state) has the
+// code -m and decodes to exactly zero.  Encode: i = rint(h qi) - m, with
+// qi = 1/qs and qc = 1.5 * 2^23 - m.
+void q16_maps(const double *range, float *qs, float *qo, float *qi, float *qc)
+{
+    for (int q = 0; q < 9; ++q) {
+        const double lo = range[2 * q], hi = range[2 * q + 1];
+        const double ideal = (hi - lo) / 65534.0;
+        int e;
+        std::frexp(ideal, &e);                   // ideal = f 2^e, f in [0.5, 1)
+        double sc = std::ldexp(std::ceil(std::ldexp(ideal, 8 - e)), e - 8);
+        const double m = std::nearbyint(0.5 * (lo + hi) / sc);
+        qs[q] = (float)sc;
+        qo[q] = (float)(sc * m);
+        qi[q] = (float)(1.0 / sc);
+        qc[q] = (float)(12582912.0 - m);
+    }
+}
+
 struct GraphKey {
     int n, parity;
     bool operator<(const GraphKey &o) const { return n != o.n ? n < o.n : parity < o.parity; }
@@ -1234,16 +1342,20 @@ struct lbm_fused {
     int bc[4] = {0, 0, 0, 0};
     int corners = 0;
     int n_links = 0, n_obs = 0, ring = 0, n_slots = 0, n_link_cells = 0;
+    float qs[9] = {}, qo[9] = {}, qi[9] = {}, qc[9] = {};   // Q16 affine maps
+    float *d_qtab = nullptr;                                // the same on the device
     void *buf[2] = {nullptr, nullptr};
-    void *ckpt = nullptr;
-    int64_t ckpt_it = -1;
+    void *ckpt[2] = {nullptr, nullptr};   // the two most recent checkpoints
+    int64_t ckpt_it[2] = {-1, -1};
     int *d_tile_slot = nullptr, *d_cell_off = nullptr, *d_link_cells = nullptr;
     uint32_t *d_cell_mask = nullptr;
     int n_perim = 0, n_perim_wg = 0;
     bool vec4 = false;
-    const double *amp = nullptr;   // separable inlets (batched cases)
+    const double *amp = nullptr;   // separable inlets (batched cases, lbm_fused_set_profiles)
     int amp_stride = 0;
     const double *prof[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    double *d_amp_own = nullptr;   // lbm_fused_set_profiles: [ring][5] amplitudes
+    double *d_prof_own[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     LinkRec *d_links = nullptr;
     int *d_obs_off = nullptr;
     int8_t *d_plan_q = nullptr;
@@ -1310,6 +1422,13 @@ Params base_params(const lbm_fused *h)
     p.link_cells = h->d_link_cells;
     p.n_link_cells = h->n_link_cells;
     p.tile_shift = h->tile_shift;
+    for (int q = 0; q < 9; ++q) {
+        p.qs[q] = h->qs[q];
+        p.qo[q] = h->qo[q];
+        p.qi[q] = h->qi[q];
+        p.qc[q] = h->qc[q];
+    }
+    p.qtab = h->d_qtab;
     return p;
 }
 
@@ -1341,7 +1460,8 @@ template <int S> void launch_step_s(const lbm_fused *h, const Params &p, hipStre
         else k_step<S, 2><<<grid, 256, 0, s>>>(p);
         break;
     case 4:
-        if constexpr (S == LBM_STORE_F32 || S == LBM_STORE_F16 || S == LBM_STORE_BF16)
+        if constexpr (S == LBM_STORE_F32 || S == LBM_STORE_F16 || S == LBM_STORE_BF16 ||
+                      S == LBM_STORE_Q16)
             k_step_w7<S, 4><<<grid, 256, 0, s>>>(p);
         else
             k_step<S, 4><<<grid, 256, 0, s>>>(p);
@@ -1372,6 +1492,7 @@ void launch_step(const lbm_fused *h, const Params &p, hipStream_t s)
     case LBM_STORE_F32: launch_step_s<LBM_STORE_F32>(h, p, s); break;
     case LBM_STORE_F16: launch_step_s<LBM_STORE_F16>(h, p, s); break;
     case LBM_STORE_BF16: launch_step_s<LBM_STORE_BF16>(h, p, s); break;
+    case LBM_STORE_Q16: launch_step_s<LBM_STORE_Q16>(h, p, s); break;
     default: launch_step_s<LBM_STORE_F16E>(h, p, s); break;
     }
 }
@@ -1383,6 +1504,7 @@ template <int MODE> void launch_fused(const lbm_fused *h, const Params &p, hipSt
     case LBM_STORE_F32: launch_fused_s<LBM_STORE_F32, MODE>(h, p, s); break;
     case LBM_STORE_F16: launch_fused_s<LBM_STORE_F16, MODE>(h, p, s); break;
     case LBM_STORE_BF16: launch_fused_s<LBM_STORE_BF16, MODE>(h, p, s); break;
+    case LBM_STORE_Q16: launch_fused_s<LBM_STORE_Q16, MODE>(h, p, s); break;
     default: launch_fused_s<LBM_STORE_F16E, MODE>(h, p, s); break;
     }
 }
@@ -1396,6 +1518,7 @@ void launch_drag_only(const lbm_fused *h, const Params &p, hipStream_t s)
     case LBM_STORE_F32: k_drag_only<LBM_STORE_F32><<<grid, 256, 0, s>>>(p); break;
     case LBM_STORE_F16: k_drag_only<LBM_STORE_F16><<<grid, 256, 0, s>>>(p); break;
     case LBM_STORE_BF16: k_drag_only<LBM_STORE_BF16><<<grid, 256, 0, s>>>(p); break;
+    case LBM_STORE_Q16: k_drag_only<LBM_STORE_Q16><<<grid, 256, 0, s>>>(p); break;
     default: k_drag_only<LBM_STORE_F16E><<<grid, 256, 0, s>>>(p); break;
     }
 }
@@ -1541,9 +1664,11 @@ void free_all(lbm_fused *h)
 {
     for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
     h->graphs.clear();
-    void *ptrs[] = {h->buf[0], h->buf[1], h->ckpt, h->d_tile_slot, h->d_cell_off, h->d_cell_mask,
+    void *ptrs[] = {h->buf[0], h->buf[1], h->ckpt[0], h->ckpt[1], h->d_tile_slot, h->d_cell_off, h->d_cell_mask,
                     h->d_link_cells, h->d_links, h->d_obs_off, h->d_plan_q, h->d_drag_ring,
-                    h->d_drag_out, h->d_pool, h->d_inlet_off, h->d_base, h->d_zero, h->d_part};
+                    h->d_drag_out, h->d_pool, h->d_inlet_off, h->d_base, h->d_zero, h->d_part,
+                    h->d_qtab, h->d_amp_own, h->d_prof_own[0], h->d_prof_own[1],
+                    h->d_prof_own[2], h->d_prof_own[3], h->d_prof_own[4]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
@@ -1564,7 +1689,17 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     if (d->nx < 4 || d->ny < 4) return set_error(LBM_ERR_ARG, "nx and ny must be >= 4");
     if ((long long)d->nx * d->ny * 9 >= (1LL << 31))
         return set_error(LBM_ERR_ARG, "lattice too large for 32-bit population indices");
-    if (d->storage < 0 || d->storage > LBM_STORE_F16E) return set_error(LBM_ERR_ARG, "bad storage");
+    if (d->storage < 0 || d->storage > LBM_STORE_Q16) return set_error(LBM_ERR_ARG, "bad storage");
+    if (d->storage == LBM_STORE_Q16) {
+        if (!d->q16_range_host)
+            return set_error(LBM_ERR_ARG, "Q16 storage needs q16_range_host");
+        for (int q = 0; q < 9; ++q) {
+            const double lo = d->q16_range_host[2 * q], hi = d->q16_range_host[2 * q + 1];
+            if (!(hi > lo) || !(hi - lo < 1.0) || !(lo > -1.0) || !(hi < 1.0))
+                return set_error(LBM_ERR_ARG, "Q16 range of direction " + std::to_string(q) +
+                                                  " must satisfy -1 < lo < hi < 1");
+        }
+    }
     if (d->ring < 2) return set_error(LBM_ERR_ARG, "ring must be >= 2");
     if (d->n_links < 0 || (d->n_links > 0 && (!d->links_host || !d->ibb_host)))
         return set_error(LBM_ERR_ARG, "bad link arrays");
@@ -1584,10 +1719,12 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     h->N = d->nx * d->ny;
     h->storage = d->storage;
     h->cpt = d->cells_per_thread;
+    if (d->storage == LBM_STORE_Q16) q16_maps(d->q16_range_host, h->qs, h->qo, h->qi, h->qc);
     // default nodes per work-item, the fastest on the 8192x2048 obstacle array:
-    // FP16/BF16 4, f16e 1 (176 against 181 us at 4), FP32/FP64 2
+    // FP16/BF16/Q16 4, f16e 1 (176 against 181 us at 4), FP32/FP64 2
     if (h->cpt == 0)
-        h->cpt = (d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16) ? 4
+        h->cpt = (d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16 ||
+                  d->storage == LBM_STORE_Q16) ? 4
                  : d->storage == LBM_STORE_F16E                               ? 1
                                                                               : 2;
     if (h->cpt != 1 && h->cpt != 2 && h->cpt != 4 && h->cpt != 8)
@@ -1615,9 +1752,7 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     // perimeter node otherwise
     const bool tile_walls = !h->vec4 && (d->storage == LBM_STORE_F64 ||
                                          d->storage == LBM_STORE_F32 ||
-                                         ((d->storage == LBM_STORE_F16 ||
-                                           d->storage == LBM_STORE_BF16 ||
-                                           d->storage == LBM_STORE_F16E) && d->ny >= 64));
+                                         (is16(d->storage) && d->ny >= 64));
     h->n_perim_wg = tile_walls ? 2 * d->ny : h->n_perim;
     h->magic = (uint32_t)((1ULL << 32) / (uint64_t)d->ny) + 1u;
     h->om_p = d->om_p;
@@ -1656,6 +1791,23 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
         hipMemset(h->d_base, 0, sizeof(int)) != hipSuccess) {
         free_all(h.get());
         return set_error(LBM_ERR_HIP, "hipMemset failed");
+    }
+    if (d->storage == LBM_STORE_Q16) {
+        float tab[36];
+        for (int q = 0; q < 9; ++q) {
+            tab[q] = h->qs[q];
+            tab[9 + q] = h->qo[q];
+            tab[18 + q] = h->qi[q];
+            tab[27 + q] = h->qc[q];
+        }
+        if ((e = dmalloc(h.get(), &h->d_qtab, sizeof(tab)))) {
+            free_all(h.get());
+            return e;
+        }
+        if (hipMemcpy(h->d_qtab, tab, sizeof(tab), hipMemcpyHostToDevice) != hipSuccess) {
+            free_all(h.get());
+            return set_error(LBM_ERR_HIP, "hipMemcpy failed");
+        }
     }
     *out = h.release();
     return LBM_OK;
@@ -1716,6 +1868,58 @@ extern "C" int lbm_fused_set_inlets(lbm_fused *h, int row0, int nrows, const int
         const int len = std::min(nrows - r, h->ring - row);
         LBM_HIP_TRY(hipMemcpy(h->d_inlet_off + 5 * row, off_host + 5 * r, 5 * sizeof(int) * len,
                               hipMemcpyHostToDevice));
+        r += len;
+    }
+    return LBM_OK;
+}
+
+extern "C" int lbm_fused_set_profiles(lbm_fused *h, const double *const *profile_host,
+                                      void *stream)
+{
+    if (!h || !profile_host) return set_error(LBM_ERR_ARG, "lbm_fused_set_profiles: bad argument");
+    if (h->amp && !h->d_amp_own)
+        return set_error(LBM_ERR_ARG, "lbm_fused_set_profiles: the wall inputs of a batch case "
+                                      "are the batch's (lbm_batch_set_amplitudes)");
+    // queued launches may still read the previous profiles
+    LBM_HIP_TRY(hipStreamSynchronize(S(stream)));
+    const int len[5] = {2 * h->nx, 2 * h->ny, 2 * h->nx, 2 * h->ny, h->ny};
+    if (!h->d_amp_own) {
+        if (int e = dmalloc(h, &h->d_amp_own, sizeof(double) * 5 * (size_t)h->ring)) return e;
+        LBM_HIP_TRY(hipMemset(h->d_amp_own, 0, sizeof(double) * 5 * (size_t)h->ring));
+        for (int sd = 0; sd < 5; ++sd)
+            if (int e = dmalloc(h, &h->d_prof_own[sd], sizeof(double) * len[sd])) return e;
+    }
+    for (int sd = 0; sd < 5; ++sd) {
+        if (profile_host[sd])
+            LBM_HIP_TRY(hipMemcpy(h->d_prof_own[sd], profile_host[sd], sizeof(double) * len[sd],
+                                  hipMemcpyHostToDevice));
+        else
+            LBM_HIP_TRY(hipMemset(h->d_prof_own[sd], 0, sizeof(double) * len[sd]));
+        h->prof[sd] = h->d_prof_own[sd];
+    }
+    h->amp = h->d_amp_own;
+    h->amp_stride = 5;
+    // captured launches hold the previous inlet pointers
+    for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+    h->graphs.clear();
+    return LBM_OK;
+}
+
+extern "C" int lbm_fused_set_amplitudes(lbm_fused *h, int64_t row0, int nrows,
+                                        const double *amp_host, void *stream)
+{
+    if (!h || row0 < 0 || nrows < 0 || nrows > h->ring || (nrows && !amp_host))
+        return set_error(LBM_ERR_ARG, "lbm_fused_set_amplitudes: bad argument");
+    if (!h->d_amp_own)
+        return set_error(LBM_ERR_ARG, "lbm_fused_set_amplitudes: call lbm_fused_set_profiles first");
+    // queued launches may still read the rows being replaced
+    LBM_HIP_TRY(hipStreamSynchronize(S(stream)));
+    int r = 0;
+    while (r < nrows) {
+        const int row = row_of(h, row0 + r);
+        const int len = std::min(nrows - r, h->ring - row);
+        LBM_HIP_TRY(hipMemcpy(h->d_amp_own + 5 * (size_t)row, amp_host + 5 * (size_t)r,
+                              5 * sizeof(double) * len, hipMemcpyHostToDevice));
         r += len;
     }
     return LBM_OK;
@@ -1874,19 +2078,25 @@ extern "C" int lbm_fused_save(lbm_fused *h, int64_t it, void *stream)
 {
     if (!h || it < 0) return set_error(LBM_ERR_ARG, "lbm_fused_save: bad argument");
     const size_t sb = state_bytes(h->storage, h->N);
-    if (!h->ckpt)
-        if (int e = dmalloc(h, &h->ckpt, sb)) return e;
-    LBM_HIP_TRY(hipMemcpyAsync(h->ckpt, h->buf[it & 1], sb, hipMemcpyDeviceToDevice, S(stream)));
-    h->ckpt_it = it;
+    // the slot already holding iteration it, else the older of the two
+    int k = h->ckpt_it[0] == it ? 0 : h->ckpt_it[1] == it ? 1
+                                      : (h->ckpt_it[0] <= h->ckpt_it[1] ? 0 : 1);
+    if (!h->ckpt[k])
+        if (int e = dmalloc(h, &h->ckpt[k], sb)) return e;
+    LBM_HIP_TRY(hipMemcpyAsync(h->ckpt[k], h->buf[it & 1], sb, hipMemcpyDeviceToDevice,
+                               S(stream)));
+    h->ckpt_it[k] = it;
     return LBM_OK;
 }
 
 extern "C" int lbm_fused_restore(lbm_fused *h, int64_t it, void *stream)
 {
-    if (!h || it < 0 || it != h->ckpt_it || !h->ckpt)
+    const int k = !h ? -1 : h->ckpt_it[0] == it ? 0 : h->ckpt_it[1] == it ? 1 : -1;
+    if (!h || it < 0 || k < 0 || !h->ckpt[k])
         return set_error(LBM_ERR_ARG, "lbm_fused_restore: no checkpoint of that iteration");
     const size_t sb = state_bytes(h->storage, h->N);
-    LBM_HIP_TRY(hipMemcpyAsync(h->buf[it & 1], h->ckpt, sb, hipMemcpyDeviceToDevice, S(stream)));
+    LBM_HIP_TRY(hipMemcpyAsync(h->buf[it & 1], h->ckpt[k], sb, hipMemcpyDeviceToDevice,
+                               S(stream)));
     return LBM_OK;
 }
 
@@ -1909,13 +2119,16 @@ extern "C" int lbm_mass(lbm_fused *h, int64_t it, double *out, void *stream)
         if (int e = dmalloc(h, &h->d_part, sizeof(double) * (nb + 1))) return e;
         h->n_part = nb;
     }
-    const void *b = h->buf[it & 1];
+    Params p = base_params(h);
+    p.src = h->buf[it & 1];
+    double *part = h->d_part;
     switch (h->storage) {
-    case LBM_STORE_F64: k_mass_partial<LBM_STORE_F64><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
-    case LBM_STORE_F32: k_mass_partial<LBM_STORE_F32><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
-    case LBM_STORE_F16: k_mass_partial<LBM_STORE_F16><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
-    case LBM_STORE_BF16: k_mass_partial<LBM_STORE_BF16><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
-    default: k_mass_partial<LBM_STORE_F16E><<<nb, 256, 0, S(stream)>>>(b, h->N, h->d_part); break;
+    case LBM_STORE_F64: k_mass_partial<LBM_STORE_F64><<<nb, 256, 0, S(stream)>>>(p, part); break;
+    case LBM_STORE_F32: k_mass_partial<LBM_STORE_F32><<<nb, 256, 0, S(stream)>>>(p, part); break;
+    case LBM_STORE_F16: k_mass_partial<LBM_STORE_F16><<<nb, 256, 0, S(stream)>>>(p, part); break;
+    case LBM_STORE_BF16: k_mass_partial<LBM_STORE_BF16><<<nb, 256, 0, S(stream)>>>(p, part); break;
+    case LBM_STORE_Q16: k_mass_partial<LBM_STORE_Q16><<<nb, 256, 0, S(stream)>>>(p, part); break;
+    default: k_mass_partial<LBM_STORE_F16E><<<nb, 256, 0, S(stream)>>>(p, part); break;
     }
     LBM_LAUNCH_CHECK();
     k_sum_serial<<<1, 64, 0, S(stream)>>>(h->d_part, nb, mass_base(h->storage, h->N),
@@ -1990,6 +2203,7 @@ void launch_batch(const lbm_batch *b, int k, int parity, hipStream_t s)
     case LBM_STORE_F32: launch_batch_t<LBM_STORE_F32>(b, k, parity, s); break;
     case LBM_STORE_F16: launch_batch_t<LBM_STORE_F16>(b, k, parity, s); break;
     case LBM_STORE_BF16: launch_batch_t<LBM_STORE_BF16>(b, k, parity, s); break;
+    case LBM_STORE_Q16: launch_batch_t<LBM_STORE_Q16>(b, k, parity, s); break;
     default: launch_batch_t<LBM_STORE_F16E>(b, k, parity, s); break;
     }
 }
@@ -2041,6 +2255,7 @@ extern "C" int lbm_batch_create(const lbm_batch_desc *d, lbm_batch **out)
         fd.ring = d->ring;
         fd.cells_per_thread = d->cells_per_thread;
         fd.vector_width = vw;
+        fd.q16_range_host = d->q16_range_host;
         lbm_fused *h = nullptr;
         if (int e = lbm_fused_create(&fd, &h)) {
             batch_free(b.get());

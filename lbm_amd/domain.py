@@ -12,12 +12,35 @@ import numpy as np
 from lbm_amd import _lib
 
 
+_C = np.array([[0, 0], [1, 0], [-1, 0], [0, 1], [0, -1], [1, 1], [-1, -1], [-1, 1], [1, -1]])
+_W = np.array([4. / 9.] + [1. / 9.] * 4 + [1. / 36.] * 4)
+
+
+def q16_range(u_bound, rho_bound=0.05, margin=1.25):
+    """Per-direction range [lo, hi] of h_q = f_q - w_q for the Q16 storage form:
+    the equilibrium deviation w_q (drho + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u.u))
+    over |u| <= u_bound and |rho - 1| <= rho_bound, widened about its centre by
+    ``margin`` for the non-equilibrium part.  Returns float64 [9][2]."""
+    U, D = float(u_bound), float(rho_bound)
+    out = np.zeros((9, 2))
+    for q in range(9):
+        a = float(np.hypot(*_C[q])) * U            # largest |c_q . u|
+        top = 3.0 * a + 4.5 * a * a                # largest 3 cu + 4.5 cu^2 (u.u >= 0)
+        bot = -3.0 * a - 1.5 * U * U               # smallest 3 cu + 4.5 cu^2 - 1.5 u.u
+        lo = -D + (1.0 + D) * min(bot, 0.0)
+        hi = D + (1.0 + D) * max(top, 0.0)
+        c, r = 0.5 * (lo + hi), 0.5 * (hi - lo) * float(margin)
+        out[q] = (_W[q] * (c - r), _W[q] * (c + r))
+    return out
+
+
 class FusedDomain:
     def __init__(self, nx, ny, om_p, om_m, bc, obstacles_links=(), ibb=False,
                  obstacles_first=True, storage="f64", ring=1002, cells_per_thread=0,
-                 vector_width=0):
+                 vector_width=0, q16_range=None):
         """bc: four LBM_BC_* codes (bottom, left, top, right).
-        obstacles_links: sequence of (boundary int[nb,3], ibb float[nb] or None)."""
+        obstacles_links: sequence of (boundary int[nb,3], ibb float[nb] or None).
+        q16_range: [9][2] range of f_q - w_q for storage "q16" (see q16_range())."""
         self.L = _lib.load()
         self.nx, self.ny = int(nx), int(ny)
         self.storage = storage
@@ -54,6 +77,9 @@ class FusedDomain:
         d.ring = self.ring
         d.cells_per_thread = int(cells_per_thread)
         d.vector_width = int(vector_width)
+        if q16_range is not None:
+            self.q16 = np.ascontiguousarray(q16_range, dtype=np.float64).reshape(9, 2)
+            d.q16_range_host = self.q16.ctypes.data_as(_lib.c_dp)
         h = _lib.c_vp()
         _lib.check(self.L.lbm_fused_create(ctypes.byref(d), ctypes.byref(h)), "lbm_fused_create")
         self.h = h
@@ -87,6 +113,24 @@ class FusedDomain:
         arrays, each checked against this lattice's side shapes."""
         pool, offs = pack_inlets(rows, self.nx, self.ny)
         self.set_inlets(row0, offs, pool)
+
+    def set_profiles(self, profiles):
+        """Separable wall inputs: side s at iteration it = amp[it][s] * profiles[s]
+        (five arrays: u_bot [2][nx], u_left [2][ny], u_top [2][nx], u_right
+        [2][ny], rho_right [ny]; None = zeros)."""
+        keep = [None if p is None else np.ascontiguousarray(p, dtype=np.float64) for p in profiles]
+        arr = (_lib.c_dp * 5)(*[(_lib.c_dp() if p is None else p.ctypes.data_as(_lib.c_dp))
+                                for p in keep])
+        _lib.check(self.L.lbm_fused_set_profiles(self.h, arr, _lib.stream_ptr()),
+                   "lbm_fused_set_profiles")
+
+    def set_amplitudes(self, row0, amp):
+        """Rows row0 .. of the amplitude table: amp [n][5]."""
+        a = np.ascontiguousarray(amp, dtype=np.float64).reshape(-1, 5)
+        _lib.check(self.L.lbm_fused_set_amplitudes(self.h, int(row0), len(a),
+                                                   a.ctypes.data_as(_lib.c_vp),
+                                                   _lib.stream_ptr()),
+                   "lbm_fused_set_amplitudes")
 
     def init(self, g, rho=None, u=None):
         _lib.check(self.L.lbm_init(self.h, self._p(g), self._p(rho), self._p(u),

@@ -285,6 +285,16 @@ class lattice:
         Cy = -2.0 * fy / (R_ref * L_ref * U_ref ** 2)
         return Cx, Cy
 
+    def drag_lift_chunk(self, obs, R_ref, U_ref, L_ref):
+        """drag_lift of every iteration of the chunk run() is replaying (for an
+        app's replay_chunk): arrays (Cx, Cy), each element computed as
+        drag_lift computes it."""
+        if self._replay is None or not hasattr(self._replay, "drag_sums_chunk"):
+            raise RuntimeError("drag_lift_chunk is for replay_chunk hooks during run()")
+        fx, fy = self._replay.drag_sums_chunk(obs)
+        den = (R_ref * L_ref * U_ref ** 2)
+        return -2.0 * fx / den, -2.0 * fy / den
+
     def bounce_back_obstacle(self, obstacle):
         """Plain or interpolated bounce-back on an obstacle's links (nb.py:77-117)."""
         if self._recording is not None:

@@ -24,6 +24,18 @@ the perimeter).  Underneath, iterations run in chunks on the GPU:
 Chunks start at multiples of ``app.output_freq``; an app without it runs one
 iteration per chunk.  A boundary plan the fused kernel cannot express runs the
 reference's call-by-call loop on the per-call kernels instead.
+
+Apps whose wall inputs are amplitude(it) x fixed profile (the reference's
+cavity and channel apps) declare it with ``separable_inlets(lattice)``: the
+wall arrays then come from device tables (one amplitude row per iteration,
+lbm_fused_set_amplitudes) instead of a ``set_inlets`` call and a snapshot per
+iteration; ``set_inlets`` still runs once per chunk, and its arrays must equal
+amplitude x profile bit for bit (else the run stops with an error).  Such runs
+are pipelined: chunk k+1 runs on the GPU while the hooks of chunk k replay on
+the host (a stop inside chunk k discards it and re-advances chunk k from its
+checkpoint).  An app may also replay a whole chunk at once with
+``replay_chunk(lattice, it0, it1, at)`` (vectorised observables; ``at(it)``
+marks the iteration whose hooks run), returning the stopping iteration or None.
 """
 import math
 import time
@@ -31,7 +43,7 @@ import time
 import numpy as np
 
 from lbm_amd import _lib
-from lbm_amd.domain import FusedDomain
+from lbm_amd.domain import FusedDomain, q16_range
 
 _SIDES = ("u_bot", "u_left", "u_top", "u_right", "rho_right")
 _WALLS = {"bottom": 0, "left": 1, "top": 2, "right": 3, "right_pressure": 3}
@@ -115,6 +127,45 @@ class _InletLog:
         return self.last
 
 
+def app_q16_range(app, lattice):
+    """Range of f_q - w_q for the Q16 storage form: the app's ``q16_range``
+    ([9][2]) if it sets one, else q16_range(u_bound) with the app's
+    ``q16_u_bound`` or 1.5 x its velocity scale u_lbm (the lid or inlet peak
+    speed of the reference apps).  None for the other forms."""
+    if lattice.precision != "q16":
+        return None
+    r = getattr(app, "q16_range", None)
+    if r is not None:
+        return r
+    u = getattr(app, "q16_u_bound", None)
+    if u is None:
+        u = 1.5 * float(getattr(app, "u_lbm", lattice.u_lbm))
+    return q16_range(u)
+
+
+def _separable(app, lattice):
+    """(profiles, amplitudes) of an app that declares separable wall inputs,
+    else None: profiles = five arrays (u_bot, u_left, u_top, u_right,
+    rho_right), amplitudes(it) = five factors."""
+    fn = getattr(app, "separable_inlets", None)
+    if fn is None:
+        return None
+    # only when set_inlets is the one the declaration describes (a subclass
+    # overriding set_inlets alone keeps the per-iteration path)
+    owner = next(c for c in type(app).__mro__ if "separable_inlets" in c.__dict__)
+    if type(app).set_inlets is not owner.__dict__.get("set_inlets"):
+        return None
+    profiles, amp = fn(lattice)
+    shapes = ((2, lattice.nx), (2, lattice.ny), (2, lattice.nx), (2, lattice.ny), (lattice.ny,))
+    prof = []
+    for p, shp in zip(profiles, shapes):
+        a = np.zeros(shp) if p is None else np.ascontiguousarray(p, dtype=np.float64)
+        if a.shape != shp:
+            raise ValueError(f"separable_inlets: profile of shape {a.shape}, expected {shp}")
+        prof.append(a)
+    return prof, amp
+
+
 class FusedRun:
     """Drives ``app`` on ``lattice`` with the fused time step."""
 
@@ -130,8 +181,12 @@ class FusedRun:
                                [(o.boundary, getattr(o, "ibb", None)) for o in obstacles],
                                ibb=bool(lattice.IBB), obstacles_first=obs_first,
                                storage=lattice.precision, ring=self.chunk + 2,
-                               cells_per_thread=cells_per_thread)
+                               cells_per_thread=cells_per_thread,
+                               q16_range=app_q16_range(app, lattice))
         self.log = _InletLog(lattice)
+        self.sep = _separable(app, lattice)
+        if self.sep is not None:
+            self.dom.set_profiles(self.sep[0])
         self.mass_history = []       # (iteration, total density in FP64) per chunk
         self.max_drift = getattr(app, "max_mass_drift", None)
         self.prev_row = None
@@ -140,6 +195,9 @@ class FusedRun:
         n = (lattice.nx, lattice.ny)
         self.rho_b = torch.empty(n, dtype=torch.float64, device="cuda")
         self.u_b = torch.empty((2,) + n, dtype=torch.float64, device="cuda")
+        # pipelined runs: moments of the first iteration of the chunk ahead
+        self.rho_s = torch.empty(n, dtype=torch.float64, device="cuda") if self.sep else None
+        self.u_s = torch.empty((2,) + n, dtype=torch.float64, device="cuda") if self.sep else None
 
     def close(self):
         self.dom.close()
@@ -150,18 +208,41 @@ class FusedRun:
         self.dom.set_inlet_rows(it0 - 1 if it0 > 0 else 0, all_rows)
         self.prev_row = rows[-1]
 
-    def _advance(self, it0, it1, macro_last):
-        """Run iterations it0..it1-1: macro of it0 into lattice rho/u, of it1-1
-        into the chunk-end buffers (or lattice rho/u when macro_last)."""
+    def _upload_sep(self, it0, it1):
+        """Amplitude rows of iterations it0-1 .. it1-1 (separable inlets)."""
+        amp = self.sep[1]
+        r0 = max(it0 - 1, 0)
+        rows = np.array([amp(it) for it in range(r0, it1)], dtype=np.float64).reshape(-1, 5)
+        self.dom.set_amplitudes(r0, rows)
+
+    def _check_separable(self, it):
+        """The app's set_inlets(it) must give amplitude x profile exactly."""
+        self.app.set_inlets(self.lat, it)
+        prof, amp = self.sep
+        a = amp(it)
+        for k, name in enumerate(_SIDES):
+            ref = np.asarray(getattr(self.lat, name), dtype=np.float64)
+            if not np.array_equal(ref, a[k] * prof[k]):
+                raise RuntimeError(f"separable_inlets of {type(self.app).__name__} disagrees "
+                                   f"with its set_inlets at iteration {it} ({name})")
+
+    def _advance(self, it0, it1, macro_last, first=None):
+        """Run iterations it0..it1-1: macro of it0 into lattice rho/u (or into
+        ``first`` = (rho, u)), of it1-1 into the chunk-end buffers (or lattice
+        rho/u when macro_last)."""
         dom = self.dom
         f = self.lat._fields
-        rho, u = f["rho"].device(load=False), f["u"].device(load=False)
+        if first is None:
+            rho, u = f["rho"].device(load=False), f["u"].device(load=False)
+        else:
+            rho, u = first
         if it0 == 0:
             dom.init(f["g"].device(), rho, u)
         else:
             dom.step(it0, rho, u)
-        f["rho"].written()
-        f["u"].written()
+        if first is None:
+            f["rho"].written()
+            f["u"].written()
         if it1 - it0 >= 3:
             dom.steps(it0 + 1, it1 - it0 - 2)
         if it1 - it0 >= 2:
@@ -203,6 +284,8 @@ class FusedRun:
         return max(it1, it0 + 1)
 
     def go(self):
+        if self.sep is not None:
+            return self._go_pipelined()
         app, lat, dom = self.app, self.lat, self.dom
         it0 = 0
         while True:
@@ -234,6 +317,55 @@ class FusedRun:
             self._finish(stop_it)
             return stop_it
 
+    def _last_chunk(self, it1):
+        """True when check_stop must end the run inside [.., it1)."""
+        app = self.app
+        return getattr(app, "stop", None) == "it" and hasattr(app, "it_max") and \
+            it1 >= int(app.it_max) + 1
+
+    def _go_pipelined(self):
+        import torch
+        lat, dom = self.lat, self.dom
+        it0, it1 = 0, self._chunk_end(0)
+        self._check_separable(0)
+        self._upload_sep(it0, it1)
+        self._advance(it0, it1, macro_last=False)
+        while True:
+            if self.obstacles:
+                dom.drag_only(it1 - 1)
+                self.drag, self.drag_it0 = dom.drag_read(it0, it1 - it0), it0
+            self._check_state(it1 - 1)
+            nxt = None
+            if not self._last_chunk(it1):
+                # the next chunk runs on the GPU while this one's hooks replay
+                n1 = self._chunk_end(it1)
+                self._check_separable(it1)
+                self._upload_sep(it1, n1)
+                dom.save(it1 - 1)
+                self._advance(it1, n1, macro_last=False, first=(self.rho_s, self.u_s))
+                nxt = (it1, n1)
+            stop_it = self._replay(it0, it1)
+            if stop_it is None:
+                if nxt is None:
+                    raise RuntimeError("check_stop did not end the run at it_max")
+                f = lat._fields
+                f["rho"].device(load=False).copy_(self.rho_s)
+                f["u"].device(load=False).copy_(self.u_s)
+                f["rho"].written()
+                f["u"].written()
+                it0, it1 = nxt
+                continue
+            # stop inside this chunk: drop the chunk ahead, re-advance this one
+            torch.cuda.synchronize()
+            if it0 > 0:
+                dom.restore(it0 - 1)
+            self._upload_sep(it0, stop_it + 1)
+            self._advance(it0, stop_it + 1, macro_last=True)
+            self._finish(stop_it)
+            # the wall arrays as the reference leaves them
+            self.app.set_inlets(lat, stop_it)
+            return stop_it
+
     def _check_state(self, it):
         """Failure detection once per chunk: the FP64 density sum of the
         post-collision state of iteration ``it`` (lbm_mass) must be finite --
@@ -255,8 +387,16 @@ class FusedRun:
     def _replay(self, it0, it1):
         app, lat = self.app, self.lat
         state = _Replay(self, it0)
+        state.it1 = it1
         lat._replay = state
         try:
+            chunk_fn = getattr(app, "replay_chunk", None)
+            if chunk_fn is not None:
+                def at(it):
+                    state.it = it
+                stop = chunk_fn(lat, it0, it1, at)
+                if stop is not NotImplemented:
+                    return stop
             for it in range(it0, it1):
                 state.it = it
                 app.printings(it)
@@ -277,6 +417,15 @@ class _Replay:
 
     def drag_sums(self, obs):
         return self.run.drag_sums(obs)
+
+    def drag_sums_chunk(self, obs):
+        """(fx, fy) arrays of the iterations it0 .. it1-1 of the chunk."""
+        run = self.run
+        for k, o in enumerate(run.obstacles):
+            if o is obs:
+                rows = run.drag[self.it0 - run.drag_it0:self.it1 - run.drag_it0, k]
+                return rows[:, 0].copy(), rows[:, 1].copy()
+        raise KeyError("drag_lift on an obstacle that set_bc does not bounce back")
 
 
 def _run_per_call(lattice, app):
