@@ -32,7 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-STORE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "f16e": 2}
+STORE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "f16e": 2, "q16": 2}
 # algorithmic bytes per lattice update: nine populations read and written
 # (18 x storage bytes); f16e also reads and writes its half-pair velocity
 BYTES_PER_UPDATE = {f: 18 * b for f, b in STORE_BYTES.items()}
@@ -79,6 +79,16 @@ def inlet_rows(app, n_rows):
     return rows
 
 
+# Q16 storage: velocity bound of the range of f - w (lbm_amd.domain.q16_range),
+# in units of the case's inlet speed u_lbm
+Q16_U_FACTOR = {"array": 2.5, "turek": 1.5}
+
+
+def q16_for(case, u_lbm):
+    from lbm_amd.domain import q16_range
+    return q16_range(Q16_U_FACTOR[case] * u_lbm)
+
+
 # the obstacle-array configuration runs 10 000 iterations: the accuracy of
 # every form is measured there, whatever the number of timed iterations
 ACC_ITERATIONS = 10000
@@ -109,7 +119,8 @@ def run_form(app, lat, om_p, om_m, storage, steps, warmup, graph_len, cpt=0, vec
     bc = [_lib.BC_VELOCITY, _lib.BC_VELOCITY, _lib.BC_VELOCITY, _lib.BC_PRESSURE]
     dom = FusedDomain(nx, ny, om_p, om_m, bc, [(o.boundary, o.ibb) for o in app.obstacles],
                       ibb=True, storage=storage, ring=total + 3, cells_per_thread=cpt,
-                      vector_width=vector_width)
+                      vector_width=vector_width,
+                      q16_range=q16_for("array", app.u_lbm) if storage == "q16" else None)
     pool, offs = pack_inlets(inlet_rows(app, total + 2))
     dom.set_inlets(0, offs, pool)
     w = torch.tensor([4. / 9.] + [1. / 9.] * 4 + [1. / 36.] * 4, dtype=torch.float64)
@@ -148,12 +159,14 @@ def run_form(app, lat, om_p, om_m, storage, steps, warmup, graph_len, cpt=0, vec
     mass = dom.mass(last)
     dom.epilogue(last, g, gup, rho, u)
     ux = u.cpu().numpy()
+    speed = float(np.sqrt((ux ** 2).sum(axis=0)).max())
     del g, gup, rho, u, g0
     dbytes = dom.device_bytes()
     dom.close()
     torch.cuda.synchronize()
     mlups = nx * ny * steps / (ms * 1e-3) / 1e6
     return dict(ms=ms, wall=wall, mlups=mlups, drag=drag, u=ux, mass=mass, device_bytes=dbytes,
+                max_speed=speed,
                 n_links=int(sum(len(o.boundary) for o in app.obstacles)), iterations=total)
 
 
@@ -265,7 +278,8 @@ def run_turek(app, o, prof, storage, iterations):
                 profiles=[None, prof, None, None, np.ones(app.ny)], links=o.boundary,
                 ibb=o.ibb, ibb_on=np.ones(len(o.boundary), np.uint8),
                 obs_offsets=[0, len(o.boundary)])
-    b = Batch([case], storage=storage, ring=iterations + 2)
+    b = Batch([case], storage=storage, ring=iterations + 2,
+              q16_range=q16_for("turek", app.u_lbm) if storage == "q16" else None)
     amp = np.zeros((iterations + 1, 1, 5))
     for it in range(iterations + 1):
         ret = (1.0 - math.exp(-it ** 2 / (2.0 * app.sigma ** 2)))
@@ -338,6 +352,31 @@ def section_turek(forms, iterations, log):
         log(f"# turek200 {f}: {entry}")
     out["config"] = "turek200: 1073x200, IBB cylinder D=48, u_max=0.05, Re=100, %d iterations" \
         % iterations
+    return out
+
+
+def section_dropin(iterations, log):
+    """The Turek channel through the drop-in lattice/run -- the turek200 app's
+    own hooks (drag/lift, moving averages, drag_lift file every iteration),
+    wall inputs from device amplitude tables, chunks pipelined with the host
+    replay -- timed over the run loop (lattice.loop_time), FP64."""
+    import tempfile
+    from lbm_amd.app.configs import turek200
+    from lbm_amd.lattice import lattice
+    from lbm_amd.run import run
+    app = turek200("f64", iterations=iterations)
+    app.verbose = False
+    app.write_png = False
+    app.results_dir = tempfile.mkdtemp(prefix="lbm_amd_dropin_") + "/"
+    lat = lattice(app)
+    last = run(lat, app)
+    loop = lat.loop_time
+    out = dict(config="turek200 app through lbm_amd.lattice + lbm_amd.run, FP64, %d iterations"
+                      % (last + 1),
+               iterations=last + 1, loop_s=round(loop, 3),
+               mlups=round(app.nx * app.ny * (last + 1) / loop / 1e6, 1),
+               host_replay="chunked (turek.replay_chunk), overlapped with the next chunk")
+    log(f"# dropin: {out}")
     return out
 
 
@@ -464,7 +503,7 @@ def section_adjoint(log):
     return out
 
 
-FORM_OF_STORE = {"0": "f64", "1": "f32", "2": "f16", "3": "bf16", "4": "f16e"}
+FORM_OF_STORE = {"0": "f64", "1": "f32", "2": "f16", "3": "bf16", "4": "f16e", "5": "q16"}
 
 
 def summarize_profile(pdir, nodes, copy_bytes=2 << 30):
@@ -578,7 +617,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--forms", default="f16,f64,f32,bf16,f16e")
+    ap.add_argument("--forms", default="f16,f64,f32,bf16,f16e,q16")
     ap.add_argument("--graph-len", type=int, default=500)
     ap.add_argument("--cpt", type=int, default=0)
     ap.add_argument("--vector-width", type=int, default=0,
@@ -590,7 +629,7 @@ def main():
                          "are compared with FP64 (the configuration's 10 000; 0 = only the "
                          "timed ones, for profiling runs)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sections", default="array,cavity,turek,batch,adjoint",
+    ap.add_argument("--sections", default="array,cavity,dropin,turek,batch,adjoint",
                     help="array (8192x2048 obstacle array, the headline), cavity (200x200), "
                          "turek (cylinder), batch (256 cavities), adjoint")
     ap.add_argument("--turek-iterations", type=int, default=200000)
@@ -627,7 +666,7 @@ def main():
         gbs = r["mlups"] * 1e6 * bpu / 1e9
         entry = {"mlups": round(r["mlups"], 1), "us_per_iteration": round(r["ms"] / args.steps * 1e3, 2),
                  "algorithmic_GBps": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-                 "bytes_per_update": bpu, "mass": r["mass"],
+                 "bytes_per_update": bpu, "mass": r["mass"], "max_speed": r["max_speed"],
                  "accuracy_iterations": r["iterations"]}
         if ref is not None and f != "f64":
             # time-averaged momentum-exchange force of each obstacle over the last
@@ -670,9 +709,14 @@ def main():
     extra = {}
     if "cavity" in sections:
         extra["cavity"] = section_cavity(20001, log)
+    if "dropin" in sections:
+        extra["dropin"] = section_dropin(args.turek_iterations, log)
     if "turek" in sections:
-        extra["turek"] = section_turek(["f64", "f32", "f16", "f16e", "bf16"],
+        extra["turek"] = section_turek(["f64", "f32", "f16", "f16e", "bf16", "q16"],
                                        args.turek_iterations, log)
+    if "dropin" in extra and "turek" in extra and "f64" in extra["turek"]:
+        extra["dropin"]["vs_batch_path_f64"] = round(
+            extra["dropin"]["mlups"] / extra["turek"]["f64"]["mlups"], 3)
     if "batch" in sections:
         extra["batch"] = section_batch(args.batch_iterations, log)
     if "adjoint" in sections:

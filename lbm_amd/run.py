@@ -466,6 +466,7 @@ def run(lattice, app, chunk=1000, fused=True):
             engine.close()
     else:
         last = _run_per_call(lattice, app)
-    print("# Loop time = {:f}".format(time.time() - start))
+    lattice.loop_time = time.time() - start
+    print("# Loop time = {:f}".format(lattice.loop_time))
     app.finalize(lattice)
     return last
