@@ -230,6 +230,19 @@ int lbm_drag_only(lbm_fused *h, int64_t it, void *stream);
  * Synchronises the stream. */
 int lbm_drag_read(lbm_fused *h, int64_t it0, int n, double *out_host, void *stream);
 
+/* Allocation-free variant of lbm_drag_read.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as lbm_drag_read, but the per-obstacle sums are
+ * staged in a fixed device array of the library (LBM_NOALLOC_DRAG_MAX
+ * doubles, 8 MiB, one per GPU, held until the process ends) instead of a
+ * buffer lbm_drag_read allocates and grows: n * n_obs * 2 must not exceed
+ * LBM_NOALLOC_DRAG_MAX (else LBM_ERR_ARG and nothing is read).  Every call on
+ * one GPU uses that GPU's array, so calls that overlap on one GPU (other
+ * streams or host threads) race.  Synchronises the stream. */
+#define LBM_NOALLOC_DRAG_MAX 1048576
+int lbm_drag_read_noalloc(lbm_fused *h, int64_t it0, int n, double *out_host, void *stream);
+
 /* Checkpoint of the post-collision state of iteration it (for rollback).
  * The handle keeps the two most recent checkpoints: a save replaces the
  * older one (or the one of the same iteration); restore takes either. */

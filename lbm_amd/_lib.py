@@ -108,6 +108,7 @@ _SIGNATURES = {
     "lbm_epilogue": ([c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "lbm_drag_only": ([c_vp, c_i64, c_vp], c_int),
     "lbm_drag_read": ([c_vp, c_i64, c_int, c_vp, c_vp], c_int),
+    "lbm_drag_read_noalloc": ([c_vp, c_i64, c_int, c_vp, c_vp], c_int),
     "lbm_fused_save": ([c_vp, c_i64, c_vp], c_int),
     "lbm_fused_restore": ([c_vp, c_i64, c_vp], c_int),
     "lbm_mass": ([c_vp, c_i64, c_vp, c_vp], c_int),

@@ -1682,6 +1682,41 @@ extern "C" const char *lbm_last_error(void) { return lbm::g_last_error.c_str(); 
 
 extern "C" int64_t lbm_fused_device_bytes(const lbm_fused *h) { return h ? h->bytes : 0; }
 
+// ------------------------------------------------ allocation-free drag read --
+// Staging array of lbm_drag_read_noalloc: the per-obstacle sums go here
+// instead of the buffer lbm_drag_read allocates and grows.  One copy per GPU,
+// held until the process ends.  Host code never passes `scratch` itself (on
+// the host the name is a stand-in, not a device address): it passes the
+// address hipGetSymbolAddress returns for the current device, kept per device.
+static __device__ __align__(256) double scratch[LBM_NOALLOC_DRAG_MAX];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// Address of `scratch` on the current device, looked up once per device
+// (lbm_fused_create, or the first lbm_drag_read_noalloc call on a device whose
+// handles all came from a batch).
+int scratch_address(double **out)
+{
+    int dev = 0;
+    LBM_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(LBM_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(LBM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                              hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch_dev[dev];
+    return LBM_OK;
+}
+}  // namespace
+
 extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
 {
     if (!d || !out) return set_error(LBM_ERR_ARG, "lbm_fused_create: null argument");
@@ -1808,6 +1843,11 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
             free_all(h.get());
             return set_error(LBM_ERR_HIP, "hipMemcpy failed");
         }
+    }
+    double *stage = nullptr;   // the drag staging array's address on this device
+    if ((e = scratch_address(&stage))) {
+        free_all(h.get());
+        return e;
     }
     *out = h.release();
     return LBM_OK;
@@ -2069,6 +2109,31 @@ extern "C" int lbm_drag_read(lbm_fused *h, int64_t it0, int n, double *out, void
         h->d_drag_out);
     LBM_LAUNCH_CHECK();
     LBM_HIP_TRY(hipMemcpyAsync(out, h->d_drag_out, sizeof(double) * need, hipMemcpyDeviceToHost,
+                               S(stream)));
+    LBM_HIP_TRY(hipStreamSynchronize(S(stream)));
+    return LBM_OK;
+}
+
+extern "C" int lbm_drag_read_noalloc(lbm_fused *h, int64_t it0, int n, double *out, void *stream)
+{
+    if (!h || n < 0 || n > h->ring || it0 < 0 || (n && !out))
+        return set_error(LBM_ERR_ARG, "lbm_drag_read_noalloc: bad argument");
+    if (n == 0 || h->n_obs == 0) return LBM_OK;
+    const int64_t need = (int64_t)n * h->n_obs * 2;
+    if (need > LBM_NOALLOC_DRAG_MAX)
+        return set_error(LBM_ERR_ARG,
+                         "lbm_drag_read_noalloc: n * n_obs * 2 = " + std::to_string(need) +
+                             " exceeds the staging array (LBM_NOALLOC_DRAG_MAX = " +
+                             std::to_string(LBM_NOALLOC_DRAG_MAX) +
+                             " doubles); read fewer iterations per call");
+    double *stage = nullptr;
+    if (int e = scratch_address(&stage)) return e;
+    // k_drag_reduce writes every one of the `need` sums: nothing to clear
+    const int threads = n * h->n_obs;
+    k_drag_reduce<<<(threads + 127) / 128, 128, 0, S(stream)>>>(
+        h->d_drag_ring, h->n_links, h->ring, it0, n, h->d_obs_off, h->n_obs, h->d_plan_q, stage);
+    LBM_LAUNCH_CHECK();
+    LBM_HIP_TRY(hipMemcpyAsync(out, stage, sizeof(double) * need, hipMemcpyDeviceToHost,
                                S(stream)));
     LBM_HIP_TRY(hipStreamSynchronize(S(stream)));
     return LBM_OK;
