@@ -380,6 +380,59 @@ def section_dropin(iterations, log):
     return out
 
 
+# the reference's published Turek results (README.md, "Turek benchmark"): ny -> Re -> (Cd, Cl)
+PUBLISHED_TUREK = {100: {20: (5.7111, 0.0285), 100: (3.5409, 1.0696)},
+                   200: {20: (5.6250, 0.0212), 100: (3.2959, 1.0253)}}
+SCHAEFER_TUREK = {20: (5.5800, 0.0107), 100: (3.2300, 1.0000)}
+
+
+def section_published(log, cap=3000000):
+    """The reference's Turek app itself (lbm/src/app/turek.py: stop = 'obs', obs_cv_ct = 1e-3,
+    obs_cv_nb = 1000) through the drop-in lattice/run, FP64, at ny = 100 and 200, Re = 20
+    (2D-1) and 100 (2D-2), against the values the reference's README publishes: the final
+    Cd/Cl for 2D-1, the maxima of |Cd| and |Cl| for 2D-2 (the code's Cd is negative: drag
+    along +x with links pointing into the solid; magnitudes are compared)."""
+    import tempfile
+    from lbm_amd.app.turek import turek
+    from lbm_amd.lattice import lattice
+    from lbm_amd.run import run
+    out = {}
+    for ny, by_re in PUBLISHED_TUREK.items():
+        for re_, (cd_pub, cl_pub) in by_re.items():
+            app = turek()
+            app.L_lbm = ny
+            app.Re_lbm = float(re_)
+            app.compute_lbm_parameters()
+            app.obstacles[0].size = app.r_cyl
+            app.it_max = cap
+            app.verbose = False
+            app.write_png = False
+            app.results_dir = tempfile.mkdtemp(prefix="lbm_amd_turek_") + "/"
+            lat = lattice(app)
+            last = run(lat, app)
+            cd = np.abs(app.drag_buff.buff[2:])
+            cl = np.abs(app.lift_buff.buff[2:])
+            tail = slice(int(len(cd) * 4 // 5), None)
+            if re_ == 20:
+                got = (float(cd[-1]), float(cl[-1]))
+            else:
+                got = (float(cd[tail].max()), float(cl[tail].max()))
+            key = f"ny{ny}_re{re_}"
+            out[key] = dict(iterations=last + 1, converged=bool(app.drag_buff.obs_cv and
+                                                                 app.lift_buff.obs_cv),
+                            loop_s=round(lat.loop_time, 2), cd=got[0], cl=got[1],
+                            cd_published=cd_pub, cl_published=cl_pub,
+                            cd_rel_diff=(got[0] - cd_pub) / cd_pub,
+                            cl_rel_diff=(got[1] - cl_pub) / cl_pub,
+                            cd_schaefer_turek=SCHAEFER_TUREK[re_][0],
+                            cl_schaefer_turek=SCHAEFER_TUREK[re_][1],
+                            max_over_whole_run=(float(cd.max()), float(cl.max())))
+            log(f"# published {key}: {out[key]}")
+    out["config"] = ("turek app (reference defaults, stop='obs'), FP64, through lattice/run; "
+                     "2D-1: final |Cd|, |Cl|; 2D-2: max |Cd|, |Cl| over the last fifth")
+    return out
+
+
 def section_cavity(iterations, log):
     """Lid-driven cavity 200x200, FP64 (launch-bound: 40 000 nodes)."""
     import torch
@@ -631,7 +684,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sections", default="array,cavity,dropin,turek,batch,adjoint",
                     help="array (8192x2048 obstacle array, the headline), cavity (200x200), "
-                         "turek (cylinder), batch (256 cavities), adjoint")
+                         "dropin (Turek through lattice/run), turek (cylinder per form), batch "
+                         "(256 cavities), adjoint; optional: published (the reference app's "
+                         "Turek runs against its README's values)")
     ap.add_argument("--turek-iterations", type=int, default=200000)
     ap.add_argument("--batch-iterations", type=int, default=5000)
     ap.add_argument("--summarize-profile", metavar="DIR",
@@ -711,6 +766,8 @@ def main():
         extra["cavity"] = section_cavity(20001, log)
     if "dropin" in sections:
         extra["dropin"] = section_dropin(args.turek_iterations, log)
+    if "published" in sections:
+        extra["published_turek"] = section_published(log)
     if "turek" in sections:
         extra["turek"] = section_turek(["f64", "f32", "f16", "f16e", "bf16", "q16"],
                                        args.turek_iterations, log)
