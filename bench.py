@@ -8,15 +8,18 @@ height, TRT with Lambda = 1/4.  Inputs are synthetic and resident in HBM
 before the timed region.
 
 The headline value is the MLUPS of the FP16-offset storage form (f - w stored
-as IEEE half, FP32 arithmetic).  The FP64, FP32, BF16-offset and f16e forms
-(half-precision f - w - E(v) with the node velocity v in a half pair) are
+as IEEE half, FP32 arithmetic).  The FP64, FP32, BF16-offset, f16e (half-
+precision f - w - E(v) with the node velocity v in a half pair), f16e8 (the
+same with v as an 8-bit pair) and Q16 (16-bit fixed point of f - w) forms are
 timed on the same workload and reported under "forms", each with its
 fraction of the 8 TB/s HBM roofline (algorithmic bytes per lattice update:
-18 x storage bytes, 44 for f16e), its drag error against the FP64 run and its
-velocity-field L2 error.  Further sections: the 200x200 cavity, the Turek
-cylinder (shedding statistics per form), a 256-cavity batch and the adjoint.
+18 x storage bytes, 44 for f16e, 40 for f16e8), its drag error against the
+FP64 run and its velocity-field L2 error.  Further sections: the 200x200
+cavity, the Turek cylinder through the drop-in lattice/run, the Turek
+cylinder per form (shedding statistics), a 256-cavity batch and the adjoint;
+optionally the reference's Turek app against its published table.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms f16,f64,f32,bf16,f16e]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms f16,f64,...] [--sections ...]
 """
 import argparse
 import json
@@ -32,11 +35,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-STORE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "f16e": 2, "q16": 2}
+STORE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "f16e": 2, "q16": 2, "f16e8": 2}
 # algorithmic bytes per lattice update: nine populations read and written
-# (18 x storage bytes); f16e also reads and writes its half-pair velocity
+# (18 x storage bytes); f16e and f16e8 also read and write their velocity word
 BYTES_PER_UPDATE = {f: 18 * b for f, b in STORE_BYTES.items()}
 BYTES_PER_UPDATE["f16e"] = 18 * 2 + 8
+BYTES_PER_UPDATE["f16e8"] = 18 * 2 + 4
 
 
 def build_case(ny=2048, aspect=4, n_obs=64, seed=0):
@@ -556,7 +560,8 @@ def section_adjoint(log):
     return out
 
 
-FORM_OF_STORE = {"0": "f64", "1": "f32", "2": "f16", "3": "bf16", "4": "f16e", "5": "q16"}
+FORM_OF_STORE = {"0": "f64", "1": "f32", "2": "f16", "3": "bf16", "4": "f16e", "5": "q16",
+                 "6": "f16e8"}
 
 
 def summarize_profile(pdir, nodes, copy_bytes=2 << 30):
@@ -670,7 +675,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--forms", default="f16,f64,f32,bf16,f16e,q16")
+    ap.add_argument("--forms", default="f16,f64,f32,bf16,f16e,f16e8,q16")
     ap.add_argument("--graph-len", type=int, default=500)
     ap.add_argument("--cpt", type=int, default=0)
     ap.add_argument("--vector-width", type=int, default=0,
@@ -769,7 +774,7 @@ def main():
     if "published" in sections:
         extra["published_turek"] = section_published(log)
     if "turek" in sections:
-        extra["turek"] = section_turek(["f64", "f32", "f16", "f16e", "bf16", "q16"],
+        extra["turek"] = section_turek(["f64", "f32", "f16", "f16e", "f16e8", "bf16", "q16"],
                                        args.turek_iterations, log)
     if "dropin" in extra and "turek" in extra and "f64" in extra["turek"]:
         extra["dropin"]["vs_batch_path_f64"] = round(

@@ -45,6 +45,9 @@ extern "C" {
                                equilibrium deviation at the node's velocity v,
                                which a tenth plane keeps as a half pair; float
                                arithmetic (fused and batched paths) */
+#define LBM_STORE_F16E8 6   /* as LBM_STORE_F16E with the velocity pair as 8-bit
+                               fixed point (steps of 2^-9, |v| <= 127/512) in a
+                               16-bit tenth plane: 40 bytes per update */
 #define LBM_STORE_Q16   5   /* 16-bit fixed point of f_q - w_q on a per-direction
                                affine map covering a caller-given range (the
                                q16_range fields); float arithmetic (fused and

@@ -14,8 +14,8 @@ LBM_ERR_ARG = -1
 LBM_ERR_HIP = -2
 LBM_ERR_UNSUPPORTED = -3
 
-STORE = {"f64": 0, "f32": 1, "f16": 2, "bf16": 3, "f16e": 4, "q16": 5}
-STORE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "f16e": 2, "q16": 2}
+STORE = {"f64": 0, "f32": 1, "f16": 2, "bf16": 3, "f16e": 4, "q16": 5, "f16e8": 6}
+STORE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "f16e": 2, "q16": 2, "f16e8": 2}
 BC_NONE, BC_VELOCITY, BC_PRESSURE = 0, 1, 2
 
 c_int = ctypes.c_int

@@ -16,7 +16,7 @@ import torch
 
 from lbm_amd import _lib
 from lbm_amd.batch import Batch, autotune, cavity_amplitudes, cavity_ensemble
-from lbm_amd.domain import FusedDomain, pack_inlets
+from lbm_amd.domain import FusedDomain, pack_inlets, q16_range
 from tests.cases import batch_case_oracle
 
 pytestmark = pytest.mark.gpu
@@ -85,7 +85,7 @@ def _single(cs, amp_c, storage, vector_width=0, cpt=0):
     obs = [(links, cs["ibb"])] if links is not None else []
     dom = FusedDomain(nx, ny, cs["om_p"], cs["om_m"], cs["bc"], obs, ibb=links is not None,
                       storage=storage, ring=IT + 2, vector_width=vector_width,
-                      cells_per_thread=cpt)
+                      cells_per_thread=cpt, q16_range=_q16(storage))
     rows = []
     profs = cs["profiles"]
     for k in range(IT + 1):
@@ -105,8 +105,13 @@ def _single(cs, amp_c, storage, vector_width=0, cpt=0):
     return out
 
 
+def _q16(storage):
+    """Q16's range of f - w: velocities up to 0.15 (lids of 0.1, channel 0.05)."""
+    return q16_range(0.15) if storage == "q16" else None
+
+
 def _batch(cases, amp, storage, **kw):
-    b = Batch(cases, storage=storage, ring=IT + 2, **kw)
+    b = Batch(cases, storage=storage, ring=IT + 2, q16_range=_q16(storage), **kw)
     b.set_amplitudes(0, amp[:IT + 1])
     b.init()
     b.steps(1, IT - 2)
@@ -184,7 +189,7 @@ def test_c4_ensemble_matches_oracle(storage):
 
 
 @pytest.mark.parametrize("storage,vw", [("f16", 1), ("f16", 4), ("bf16", 1), ("bf16", 4),
-                                        ("f16e", 1)])
+                                        ("f16e", 1), ("f16e8", 1), ("q16", 1)])
 def test_batch_16bit_equals_single_runs(storage, vw):
     """16-bit storage, one or four nodes per work-item: the four-node kernel
     needs ny % 4 == 0 and nx*ny a multiple of 1024 (one group of four per
@@ -205,7 +210,8 @@ def test_batch_16bit_equals_single_runs(storage, vw):
     ch["profiles"] = [None, left, None, None, np.ones(ny)]
     cases.append(ch)
     amp = _amp(cases, IT + 1)
-    b = Batch(cases, storage=storage, ring=IT + 2, vector_width=vw, cells_per_thread=1)
+    b = Batch(cases, storage=storage, ring=IT + 2, vector_width=vw, cells_per_thread=1,
+              q16_range=_q16(storage))
     assert b.vector_width == vw
     b.close()
     outs = _batch(cases, amp, storage, vector_width=vw, cells_per_thread=1)

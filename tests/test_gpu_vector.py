@@ -92,11 +92,11 @@ def test_vector_kernel_random_state(form):
         assert np.array_equal(v, s), its
 
 
-@pytest.mark.parametrize("cpt", [2, 4])
-def test_f16e_launch_shapes_agree(cpt):
-    print(f"case f16e cpt={cpt} obstacle array", flush=True)
-    a = array_domain_case("f16e", IT, cpt=cpt)
-    b = array_domain_case("f16e", IT, cpt=1)
+@pytest.mark.parametrize("form,cpt", [("f16e", 2), ("f16e", 4), ("f16e8", 2), ("f16e8", 4)])
+def test_f16e_launch_shapes_agree(form, cpt):
+    print(f"case {form} cpt={cpt} obstacle array", flush=True)
+    a = array_domain_case(form, IT, cpt=cpt)
+    b = array_domain_case(form, IT, cpt=1)
     for k in ("rho", "u", "g", "g_up"):
         assert np.array_equal(a[k], b[k]), k
     assert np.array_equal(a["drag"], b["drag"])
