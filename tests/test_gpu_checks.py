@@ -125,3 +125,45 @@ def test_set_inlets_rejects_offsets_past_the_pool():
     with pytest.raises(ValueError, match="u_left"):
         dom.set_inlet_rows(0, [(row[0], np.zeros((2, ny - 1)), row[2], row[3], row[4])])
     dom.close()
+
+
+def test_amplitude_tables_check_their_arguments():
+    """lbm_fused_set_amplitudes needs lbm_fused_set_profiles first; a batch
+    case's wall inputs belong to the batch."""
+    from lbm_amd.batch import Batch, cavity_ensemble
+    nx, ny = 64, 32
+    dom = FusedDomain(nx, ny, 1 / 0.6, 1 / (0.25 / 0.1 + 0.5), [_lib.BC_VELOCITY] * 4,
+                      storage="f64", ring=8)
+    with pytest.raises(_lib.LbmError, match="set_profiles first"):
+        dom.set_amplitudes(0, np.zeros((3, 5)))
+    top = np.zeros((2, nx))
+    top[0] = 1.0
+    dom.set_profiles([None, None, top, None, None])
+    dom.set_amplitudes(0, np.zeros((8, 5)))                   # a whole ring
+    with pytest.raises(_lib.LbmError):
+        dom.set_amplitudes(0, np.zeros((9, 5)))               # more rows than the ring
+    dom.close()
+    b = Batch(cavity_ensemble(n_cases=2, seed=1, n_lo=32, n_hi=40), storage="f64", ring=8)
+    with pytest.raises(_lib.LbmError, match="batch"):
+        b.case(0).set_profiles([None] * 5)
+    b.close()
+
+
+def test_run_stops_when_separable_inlets_disagree_with_set_inlets():
+    """An app whose declared amplitude x profile is not what its set_inlets
+    writes is caught at the next chunk start, not run with wrong walls."""
+    class _Lying:
+        def separable_inlets(self, lattice):
+            profiles, amp = super().separable_inlets(lattice)
+            return profiles, (lambda it: tuple(1.001 * a for a in amp(it)))
+
+        def set_inlets(self, lattice, it):
+            super().set_inlets(lattice, it)
+
+    def make():
+        app = small_cavity()
+        app.__class__ = type("lying_cavity", (_Lying, type(app)), {})
+        app.sigma = 1          # the lid is moving from iteration 1
+        return app
+    with pytest.raises(RuntimeError, match="disagrees"):
+        run_gpu(make, 60, chunk=16)
