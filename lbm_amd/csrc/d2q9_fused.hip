@@ -136,6 +136,22 @@ __device__ __forceinline__ _Float16 to_half(float v)
 // F16E: offset (in halves) of the velocity plane, 256-byte aligned
 __host__ __device__ constexpr size_t vel_offset(int N) { return ((size_t)9 * N + 127) / 128 * 128; }
 
+// 16-bit forms: a plane element through a base pointer that is uniform over
+// the work-group (kept in scalar registers) and a 32-bit byte offset shared
+// by all planes of a node, so that each access is one global load or store
+// with a scalar base and no 64-bit address arithmetic per plane (offsets of
+// 2-byte elements stay below 2^32 for any int node count).
+template <class E>
+__device__ __forceinline__ E ld_off(const void *base, uint32_t off)
+{
+    return *reinterpret_cast<const E *>(static_cast<const char *>(base) + off);
+}
+template <class E>
+__device__ __forceinline__ void st_off(void *base, uint32_t off, E v)
+{
+    *reinterpret_cast<E *>(static_cast<char *>(base) + off) = v;
+}
+
 template <> struct Store<LBM_STORE_F64> {
     using s_t = double;
     using T = double;
@@ -169,6 +185,8 @@ template <> struct Store<LBM_STORE_F32> {
 template <> struct Store<LBM_STORE_F16> {
     using s_t = _Float16;
     using T = float;
+    static __device__ __forceinline__ float dec(const Params &, int, s_t raw) { return (float)raw; }
+    static __device__ __forceinline__ s_t enc(const Params &, int, float h) { return to_half(h); }
     static __device__ __forceinline__ float ldh(const Params &p, const s_t *b, int q, size_t c)
     {
         return (float)b[(size_t)q * p.N + c];
@@ -193,6 +211,11 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float v)
 template <> struct Store<LBM_STORE_BF16> {
     using s_t = uint16_t;
     using T = float;
+    static __device__ __forceinline__ float dec(const Params &, int, s_t raw)
+    {
+        return __uint_as_float((uint32_t)raw << 16);
+    }
+    static __device__ __forceinline__ s_t enc(const Params &, int, float h) { return f32_to_bf16(h); }
     static __device__ __forceinline__ float ldh(const Params &p, const s_t *b, int q, size_t c)
     {
         return __uint_as_float((uint32_t)b[(size_t)q * p.N + c] << 16);
@@ -243,6 +266,14 @@ __device__ __forceinline__ float q16_coef(const Params &p, int row, int q)
 template <> struct Store<LBM_STORE_Q16> {
     using s_t = int16_t;
     using T = float;
+    static __device__ __forceinline__ float dec(const Params &p, int q, s_t raw)
+    {
+        return q16_decode(raw, q16_coef(p, 0, q), q16_coef(p, 1, q));
+    }
+    static __device__ __forceinline__ s_t enc(const Params &p, int q, float h)
+    {
+        return (int16_t)q16_encode(h, q16_coef(p, 2, q), q16_coef(p, 3, q));
+    }
     static __device__ __forceinline__ float ldh(const Params &p, const s_t *b, int q, size_t c)
     {
         return q16_decode(b[(size_t)q * p.N + c], q16_coef(p, 0, q), q16_coef(p, 1, q));
@@ -297,21 +328,77 @@ template <> struct Vel<LBM_STORE_F16E> {
 };
 template <> struct Vel<LBM_STORE_F16E8> {
     using w_t = uint16_t;
-    static __device__ __forceinline__ int q8(float u)
-    {
-        const float x = __builtin_fminf(__builtin_fmaxf(u * 512.0f, -127.0f), 127.0f);
-        return (int)__builtin_rintf(x);
-    }
-    static __device__ __forceinline__ w_t pack(float ux, float uy)
-    {
-        return (uint16_t)((uint32_t)(q8(ux) & 0xff) | ((uint32_t)(q8(uy) & 0xff) << 8));
-    }
-    static __device__ __forceinline__ void unpack(w_t v, float &vx, float &vy)
-    {
-        vx = (float)(int8_t)(v & 0xffu) * (1.0f / 512.0f);
-        vy = (float)(int8_t)(v >> 8) * (1.0f / 512.0f);
-    }
 };
+
+// F16E8 velocity word: a = rint(512 vx), b = rint(512 vy), clamped to
+// -127..127, as two's-complement bytes (a low).  Rounding by the magic number
+// 1.5 * 2^23 (one FMA, round to nearest even, the integer in the low bits of
+// the sum) and a median clamp; a and b come back as integer-valued floats.
+__device__ __forceinline__ uint16_t e8_pack(float ux, float uy, float &a, float &b)
+{
+    constexpr float M = 12582912.0f;   // 1.5 * 2^23
+    const float xa = __builtin_amdgcn_fmed3f(__builtin_fmaf(ux, 512.0f, M), M - 127.0f, M + 127.0f);
+    const float xb = __builtin_amdgcn_fmed3f(__builtin_fmaf(uy, 512.0f, M), M - 127.0f, M + 127.0f);
+    a = xa - M;
+    b = xb - M;
+    return (uint16_t)((__float_as_uint(xa) & 0xffu) | ((__float_as_uint(xb) & 0xffu) << 8));
+}
+
+// the two signed bytes of the word as floats (one byte-select convert each)
+__device__ __forceinline__ void e8_unpack(uint32_t w, float &a, float &b)
+{
+    a = (float)((int32_t)(w << 24) >> 24);
+    b = (float)((int32_t)(w << 16) >> 24);
+}
+
+// E_q of F16E8 on the integers (a, b) = 512 v: with cu = c_q.(a, b) and
+// pv = c_q x (a, b) (the component across c_q), v.v = (cu^2 + pv^2) / |c_q|^2
+// / 512^2 and
+//   E_q = w_q (3 c.v + 4.5 (c.v)^2 - 1.5 v.v) = cu (K2 + K1 cu) - K3 pv^2,
+//   E_0 = -K0 (a^2 + b^2),
+// K2 = 3 w / 512, K1 = (4.5 - 1.5 / |c|^2) w / 512^2, K3 = 1.5 w / |c|^2 / 512^2.
+// cu and pv are small integers (exact in float); the signs of c fold into
+// the operands.  q may be a run-time value (the link path) and then yields
+// the same operations on the same constants.
+__host__ __device__ constexpr double e8_w(int q) { return q == 0 ? 4.0 / 9.0 : (q <= 4 ? 1.0 / 9.0 : 1.0 / 36.0); }
+__host__ __device__ constexpr double e8_c2(int q) { return q <= 4 ? 1.0 : 2.0; }
+__device__ __forceinline__ float e8_k0() { return (float)(1.5 * e8_w(0) / 262144.0); }
+__device__ __forceinline__ float e8_k1(int q) { return (float)((4.5 - 1.5 / e8_c2(q)) * e8_w(q) / 262144.0); }
+__device__ __forceinline__ float e8_k2(int q) { return (float)(3.0 * e8_w(q) / 512.0); }
+__device__ __forceinline__ float e8_k3(int q) { return (float)(1.5 / e8_c2(q) * e8_w(q) / 262144.0); }
+__device__ __forceinline__ float e8_cu(int q, float a, float b)
+{
+    const float ax = CX[q] > 0 ? a : -a, by = CY[q] > 0 ? b : -b;
+    return CY[q] == 0 ? ax : (CX[q] == 0 ? by : ax + by);
+}
+__device__ __forceinline__ float e8_pv(int q, float a, float b)
+{
+    // c x v = cx b - cy a
+    const float bx = CX[q] > 0 ? b : -b, ay = CY[q] > 0 ? -a : a;
+    return CY[q] == 0 ? bx : (CX[q] == 0 ? ay : bx + ay);
+}
+// reader: h = E_q + r in one chain of fused multiply-adds
+__device__ __forceinline__ float e8_dec(int q, float a, float b, float r)
+{
+    if (q == 0) {
+        const float k = -e8_k0();
+        return __builtin_fmaf(a * k, a, __builtin_fmaf(b * k, b, r));
+    }
+    const float cu = e8_cu(q, a, b), pv = e8_pv(q, a, b);
+    const float t = __builtin_fmaf(e8_k1(q), cu, e8_k2(q));
+    return __builtin_fmaf(cu, t, __builtin_fmaf(pv * -e8_k3(q), pv, r));
+}
+// writer: E_q itself
+__device__ __forceinline__ float e8_shift(int q, float a, float b)
+{
+    if (q == 0) {
+        const float k = -e8_k0();
+        return __builtin_fmaf(a * k, a, (b * k) * b);
+    }
+    const float cu = e8_cu(q, a, b), pv = e8_pv(q, a, b);
+    const float t = __builtin_fmaf(e8_k1(q), cu, e8_k2(q));
+    return __builtin_fmaf(cu, t, (pv * -e8_k3(q)) * pv);
+}
 
 // F16E / F16E8: g_q = h_q - E_q(v) in IEEE half with the velocity word in a
 // tenth plane (256-byte aligned)
@@ -323,11 +410,21 @@ template <int S> struct StoreShifted {
     {
         return reinterpret_cast<const w_t *>(b + vel_offset(N));
     }
+    static __device__ __forceinline__ float dec(int q, s_t raw, w_t v)
+    {
+        if constexpr (S == LBM_STORE_F16E8) {
+            float a, b;
+            e8_unpack(v, a, b);
+            return e8_dec(q, a, b, (float)raw);
+        } else {
+            float vx, vy;
+            Vel<S>::unpack(v, vx, vy);
+            return eq_shift(q, vx, vy) + (float)raw;
+        }
+    }
     static __device__ __forceinline__ float ldh(const Params &p, const s_t *b, int q, size_t c)
     {
-        float vx, vy;
-        Vel<S>::unpack(vel(b, p.N)[c], vx, vy);
-        return eq_shift(q, vx, vy) + (float)b[(size_t)q * p.N + c];
+        return dec(q, b[(size_t)q * p.N + c], vel(b, p.N)[c]);
     }
     static __device__ __forceinline__ T ld(const Params &p, const s_t *b, int q, size_t c)
     {
@@ -685,13 +782,26 @@ __device__ __forceinline__ void collide_store_h(const Params &p, int cell, const
     if constexpr (shifted(S)) {
         // the node's velocity word, then g_q = h_q - E_q(v) of the stored v
         using w_t = typename Vel<S>::w_t;
-        const w_t v = Vel<S>::pack(ux, uy);
-        reinterpret_cast<w_t *>(dst + vel_offset(p.N))[cell] = v;
-        float vx, vy;
-        Vel<S>::unpack(v, vx, vy);
+        const uint32_t off = (uint32_t)cell * (uint32_t)sizeof(typename St::s_t);
+        if constexpr (S == LBM_STORE_F16E8) {
+            float a, b;
+            st_off(dst + vel_offset(p.N), off, e8_pack(ux, uy, a, b));
 #pragma unroll
-        for (int q = 0; q < 9; ++q)
-            dst[(size_t)q * p.N + cell] = to_half(h[q] - eq_shift(q, vx, vy));
+            for (int q = 0; q < 9; ++q)
+                st_off(dst + (size_t)q * p.N, off, to_half(h[q] - e8_shift(q, a, b)));
+        } else {
+            const w_t v = Vel<S>::pack(ux, uy);
+            reinterpret_cast<w_t *>(dst + vel_offset(p.N))[cell] = v;
+            float vx, vy;
+            Vel<S>::unpack(v, vx, vy);
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                st_off(dst + (size_t)q * p.N, off, to_half(h[q] - eq_shift(q, vx, vy)));
+        }
+    } else if constexpr (is16(S)) {
+        const uint32_t off = (uint32_t)cell * (uint32_t)sizeof(typename St::s_t);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) st_off(dst + (size_t)q * p.N, off, St::enc(p, q, h[q]));
     } else {
 #pragma unroll
         for (int q = 0; q < 9; ++q) St::sth(p, dst, q, (size_t)q * p.N + cell, h[q]);
@@ -986,16 +1096,43 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
             rend[c] = co[64];
         }
         const int lc = (in[c] || wall[c]) ? cell : p.ny + 1;
+        if constexpr (is16(S)) {
+            // plane q shifted by the source offset as a scalar base, the
+            // node's byte offset shared by all planes.  A wall node's sources
+            // beyond the wall are read from the neighbouring column (index -1
+            // or N at the corners of the lattice: the neighbouring plane, or
+            // the state buffer's tail padding) and replaced by the wall
+            // routine.
+            const uint32_t off = (uint32_t)lc * (uint32_t)sizeof(s_t);
 #pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            // source node; a wall node's sources outside the lattice read the
-            // node itself (the value is replaced by the wall routine)
-            int sc = lc - (CX[q] * p.ny + CY[q]);
-            if (wall[c] && ((CY[q] == 1 && y == 0) || (CY[q] == -1 && y == ly))) sc = lc;
-            if constexpr (S == LBM_STORE_F64)
-                v[c][q] = St::ld(p, src, q, (size_t)sc);
-            else
-                v[c][q] = St::ldh(p, src, q, (size_t)sc);
+            for (int q = 0; q < 9; ++q) {
+                const ptrdiff_t sh = (ptrdiff_t)CX[q] * p.ny + CY[q];
+                const s_t raw = ld_off<s_t>(src + (size_t)q * p.N - sh, off);
+                if constexpr (shifted(S)) {
+                    using w_t = typename Vel<S>::w_t;
+                    const w_t *vb = St::vel(src, p.N) - sh;
+                    w_t w;
+                    if constexpr (sizeof(w_t) == sizeof(s_t))
+                        w = ld_off<w_t>(vb, off);
+                    else
+                        w = vb[(size_t)lc];
+                    v[c][q] = St::dec(q, raw, w);
+                } else {
+                    v[c][q] = St::dec(p, q, raw);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                // source node; a wall node's sources outside the lattice read
+                // the node itself (the value is replaced by the wall routine)
+                int sc = lc - (CX[q] * p.ny + CY[q]);
+                if (wall[c] && ((CY[q] == 1 && y == 0) || (CY[q] == -1 && y == ly))) sc = lc;
+                if constexpr (S == LBM_STORE_F64)
+                    v[c][q] = St::ld(p, src, q, (size_t)sc);
+                else
+                    v[c][q] = St::ldh(p, src, q, (size_t)sc);
+            }
         }
     }
 #pragma unroll
@@ -1339,12 +1476,15 @@ __global__ void k_set_int(int *p, int v)
 // ------------------------------------------------------------------ host --
 size_t store_bytes(int s) { return s == LBM_STORE_F64 ? 8 : (s == LBM_STORE_F32 ? 4 : 2); }
 
-// bytes of one state buffer (nine planes; F16E adds its velocity plane)
+// bytes of one state buffer (nine planes; F16E adds its velocity plane) and
+// 256 bytes of tail padding: the step's loads for the top wall node next to
+// the right column read one element past the last plane (a value the wall
+// routine replaces)
 size_t state_bytes(int s, int N)
 {
-    if (s == LBM_STORE_F16E) return 2 * vel_offset(N) + 4 * (size_t)N;
-    if (s == LBM_STORE_F16E8) return 2 * vel_offset(N) + 2 * (size_t)N;
-    return store_bytes(s) * 9 * (size_t)N;
+    if (s == LBM_STORE_F16E) return 2 * vel_offset(N) + 4 * (size_t)N + 256;
+    if (s == LBM_STORE_F16E8) return 2 * vel_offset(N) + 2 * (size_t)N + 256;
+    return store_bytes(s) * 9 * (size_t)N + 256;
 }
 
 // Q16 affine maps from the range [lo, hi] of h_q per direction: the codes
