@@ -1240,6 +1240,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     step_launch<S, CPT>(p);
 }
 
+// And with 8 waves per SIMD (64 VGPRs): f16e8 with one node per work-item
+// fits without spilling (66 -> 64 VGPRs; 171.6 -> 163.8 us on the 8192x2048
+// obstacle array).
+template <int S, int CPT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_step_w8(Params p)
+{
+    step_launch<S, CPT>(p);
+}
+
 // 16-bit storage forms, four consecutive nodes per work-item: every plane is
 // read and written with aligned 8-byte accesses.  A plane shifted by one node
 // in y (cy = +-1) takes its missing element from the neighbouring work-item
@@ -1637,7 +1646,10 @@ template <int S> void launch_step_s(const lbm_fused *h, const Params &p, hipStre
 {
     const int grid = h->n_tiles + perim_groups(h->n_perim_wg);
     switch (h->cpt) {
-    case 1: k_step<S, 1><<<grid, 256, 0, s>>>(p); break;
+    case 1:
+        if constexpr (S == LBM_STORE_F16E8) k_step_w8<S, 1><<<grid, 256, 0, s>>>(p);
+        else k_step<S, 1><<<grid, 256, 0, s>>>(p);
+        break;
     case 2:
         if constexpr (S == LBM_STORE_F64) k_step_w7<S, 2><<<grid, 256, 0, s>>>(p);
         else k_step<S, 2><<<grid, 256, 0, s>>>(p);
@@ -1943,12 +1955,10 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     h->cpt = d->cells_per_thread;
     if (d->storage == LBM_STORE_Q16) q16_maps(d->q16_range_host, h->qs, h->qo, h->qi, h->qc);
     // default nodes per work-item, the fastest on the 8192x2048 obstacle array:
-    // FP16/BF16/Q16 4, f16e 1 (176 against 181 us at 4), FP32/FP64 2
-    if (h->cpt == 0)
-        h->cpt = (d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16 ||
-                  d->storage == LBM_STORE_Q16) ? 4
-                 : shifted(d->storage)                                        ? 1
-                                                                              : 2;
+    // f16e and f16e8 1 (f16e8 at 8 waves per SIMD: 163.8 us against 166.6 at
+    // 2), the other forms 2 (FP16 122.0 us against 124.7 at 4, Q16 129.8
+    // against 139.2: at 2 nodes the 16-bit tiles fit 64 VGPRs, 8 waves/SIMD)
+    if (h->cpt == 0) h->cpt = shifted(d->storage) ? 1 : 2;
     if (h->cpt != 1 && h->cpt != 2 && h->cpt != 4 && h->cpt != 8)
         return set_error(LBM_ERR_ARG, "cells_per_thread must be 1, 2, 4 or 8");
     // 16-bit forms: four nodes per work-item on request, when the lattice
@@ -1956,7 +1966,7 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     // the 8192x2048 obstacle array: 122 against 111 GLUPS)
     const bool sixteen = d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16;
     if (sixteen && d->ny % 4 == 0 && d->vector_width == 4) {
-        const int cpt = (h->cpt >= 4) ? 4 : h->cpt;
+        const int cpt = (d->cells_per_thread == 0 || h->cpt >= 4) ? 4 : h->cpt;
         if ((long long)d->nx * d->ny % (1024LL * cpt) == 0) {
             h->vec4 = true;
             h->cpt = cpt;
