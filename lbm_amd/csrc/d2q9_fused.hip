@@ -64,6 +64,7 @@ struct Params {
     int nx, ny, N;
     uint32_t magic;        // floor(2^32 / ny) + 1, for node -> (x, y)
     double om_p, om_m, b0, a1, a2, a3, a4;
+    float omf_p, omf_m;    // om_p and om_m rounded to float (the float forms' rates)
     int bc_bottom, bc_left, bc_top, bc_right, corners;
     const double *pool;    // inlet side arrays
     const int *inlet_off;  // [ring][5] offsets into pool
@@ -149,7 +150,20 @@ __device__ __forceinline__ E ld_off(const void *base, uint32_t off)
 template <class E>
 __device__ __forceinline__ void st_off(void *base, uint32_t off, E v)
 {
-    *reinterpret_cast<E *>(static_cast<char *>(base) + off) = v;
+    using gchar = __attribute__((address_space(1))) char;
+    using gE = __attribute__((address_space(1))) E;
+    *(gE *)((gchar *)base + off) = v;
+}
+// A uniform pointer through readfirstlane: the compiler keeps each plane's
+// base in scalar registers instead of deriving one plane's address from the
+// previous one with per-lane 64-bit additions.
+template <class T>
+__device__ __forceinline__ T *sbase(T *ptr)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(ptr);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<T *>(((uint64_t)hi << 32) | lo);
 }
 
 template <> struct Store<LBM_STORE_F64> {
@@ -723,7 +737,7 @@ struct Coll {
 
 __device__ __forceinline__ Coll coll_consts(const Params &p)
 {
-    return Coll{(float)p.om_p, 0.5f * (float)p.om_p, 0.5f * (float)p.om_m};
+    return Coll{p.omf_p, 0.5f * p.omf_p, 0.5f * p.omf_m};
 }
 
 // In place: h (deviations of one node) -> post-collision deviations.
@@ -785,10 +799,10 @@ __device__ __forceinline__ void collide_store_h(const Params &p, int cell, const
         const uint32_t off = (uint32_t)cell * (uint32_t)sizeof(typename St::s_t);
         if constexpr (S == LBM_STORE_F16E8) {
             float a, b;
-            st_off(dst + vel_offset(p.N), off, e8_pack(ux, uy, a, b));
+            st_off(sbase(dst + vel_offset(p.N)), off, e8_pack(ux, uy, a, b));
 #pragma unroll
             for (int q = 0; q < 9; ++q)
-                st_off(dst + (size_t)q * p.N, off, to_half(h[q] - e8_shift(q, a, b)));
+                st_off(sbase(dst + (size_t)q * p.N), off, to_half(h[q] - e8_shift(q, a, b)));
         } else {
             const w_t v = Vel<S>::pack(ux, uy);
             reinterpret_cast<w_t *>(dst + vel_offset(p.N))[cell] = v;
@@ -796,12 +810,12 @@ __device__ __forceinline__ void collide_store_h(const Params &p, int cell, const
             Vel<S>::unpack(v, vx, vy);
 #pragma unroll
             for (int q = 0; q < 9; ++q)
-                st_off(dst + (size_t)q * p.N, off, to_half(h[q] - eq_shift(q, vx, vy)));
+                st_off(sbase(dst + (size_t)q * p.N), off, to_half(h[q] - eq_shift(q, vx, vy)));
         }
     } else if constexpr (is16(S)) {
         const uint32_t off = (uint32_t)cell * (uint32_t)sizeof(typename St::s_t);
 #pragma unroll
-        for (int q = 0; q < 9; ++q) st_off(dst + (size_t)q * p.N, off, St::enc(p, q, h[q]));
+        for (int q = 0; q < 9; ++q) st_off(sbase(dst + (size_t)q * p.N), off, St::enc(p, q, h[q]));
     } else {
 #pragma unroll
         for (int q = 0; q < 9; ++q) St::sth(p, dst, q, (size_t)q * p.N + cell, h[q]);
@@ -1080,7 +1094,14 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
         const int cell = base + c * 256;
-        const int x = node_x(p, cell), y = cell - x * p.ny;
+        // a wave's 64 nodes lie in one column when ny is a multiple of 64:
+        // its column index then comes from the scalar unit
+        int x;
+        if ((p.ny & 63) == 0)
+            x = node_x(p, __builtin_amdgcn_readfirstlane(cell - lane));
+        else
+            x = node_x(p, cell);
+        const int y = cell - x * p.ny;
         in[c] = cell < p.N && (unsigned)(x - 1) < (unsigned)(lx - 1) &&
                 (unsigned)(y - 1) < (unsigned)(ly - 1);
         wall[c] = p.n_perim == 2 * p.ny && cell < p.N &&
@@ -1585,6 +1606,8 @@ Params base_params(const lbm_fused *h)
     p.magic = h->magic;
     p.om_p = h->om_p;
     p.om_m = h->om_m;
+    p.omf_p = (float)h->om_p;
+    p.omf_m = (float)h->om_m;
     p.b0 = h->b0;
     p.a1 = h->a1;
     p.a2 = h->a2;
