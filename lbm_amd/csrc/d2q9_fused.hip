@@ -1073,16 +1073,15 @@ __device__ __forceinline__ void link_pass(const Params &p, int r0, int r1, int f
 // left and right columns are left to the perimeter work-groups (their nodes
 // load a valid interior node).
 template <int S, int CPT>
-__device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bool macro)
+__device__ __forceinline__ void step_tile(const Params &p, int tile, bool macro)
 {
     using St = Store<S>;
     using T = typename St::T;
     using s_t = typename St::s_t;
     const s_t *__restrict__ src = static_cast<const s_t *>(p.src);
     // link tables are read-only during the launch: the tile's slot and each
-    // wave's record range come through the scalar cache with the pull loads
+    // wave's record range come through the scalar cache, after the pull loads
     using ci32 = const __attribute__((address_space(4))) int;
-    const int slot = p.tile_slot ? ((const ci32 *)p.tile_slot)[tile] : -1;
     const int base = tile * (256 * CPT) + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int lx = p.nx - 1, ly = p.ny - 1;
@@ -1107,15 +1106,6 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
         wall[c] = p.n_perim == 2 * p.ny && cell < p.N &&
                   (unsigned)(x - 1) < (unsigned)(lx - 1) && (y == 0 || y == ly);
         bottom[c] = y == 0;
-        rbeg[c] = rend[c] = 0;
-        if (S != LBM_STORE_F64 && slot >= 0) {
-            // nodes [first, first + 64) of the wave: their records are
-            // [cell_off[first], cell_off[first + 64]) of the tile's table
-            const int lf = __builtin_amdgcn_readfirstlane((int)((threadIdx.x & ~63u) + c * 256));
-            const ci32 *co = (const ci32 *)p.cell_off + (size_t)slot * (256 * CPT + 1) + lf;
-            rbeg[c] = co[0];
-            rend[c] = co[64];
-        }
         const int lc = (in[c] || wall[c]) ? cell : p.ny + 1;
         if constexpr (is16(S)) {
             // plane q shifted by the source offset as a scalar base, the
@@ -1156,6 +1146,24 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, int row, bo
             }
         }
     }
+    // the link tables and the iteration's ring row (a device counter) only
+    // after the pull loads are issued: their dependent scalar loads then
+    // overlap the pulls' latency instead of delaying them
+    __builtin_amdgcn_sched_barrier(0);
+    const int slot = p.tile_slot ? ((const ci32 *)p.tile_slot)[tile] : -1;
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        rbeg[c] = rend[c] = 0;
+        if (S != LBM_STORE_F64 && slot >= 0) {
+            // nodes [first, first + 64) of the wave: their records are
+            // [cell_off[first], cell_off[first + 64]) of the tile's table
+            const int lf = __builtin_amdgcn_readfirstlane((int)((threadIdx.x & ~63u) + c * 256));
+            const ci32 *co = (const ci32 *)p.cell_off + (size_t)slot * (256 * CPT + 1) + lf;
+            rbeg[c] = co[0];
+            rend[c] = co[64];
+        }
+    }
+    const int row = bc_row(p);
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
         // float forms: the run's links, link-major, before the nodes branch
@@ -1232,16 +1240,30 @@ __host__ __device__ constexpr int perim_groups(int n_perim)
     return (((n_perim + 255) / 256) + 7) & ~7;
 }
 
+// One word of every 64-byte line of a parameter block, loaded in one batch:
+// the fields a kernel reads are otherwise fetched in several dependent rounds
+// of scalar loads, each of which can miss the scalar cache; on small lattices
+// those rounds sit on the critical path of every launch.
+__device__ __forceinline__ void touch_params(const Params &p)
+{
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(&p);
+    constexpr int lines = (int)((sizeof(Params) + 63) / 64);
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < lines; ++i) acc ^= w[i * 16];
+    asm volatile("" ::"s"(acc));
+}
+
 template <int S, int CPT>
 __device__ __forceinline__ void step_launch(const Params &p)
 {
-    const int row = bc_row(p);
+    touch_params(p);
     const bool macro = p.rho_out != nullptr;
     const int np = perim_groups(p.n_perim);
     if ((int)blockIdx.x < np)
-        step_perimeter<S, true>(p, (int)blockIdx.x, row, macro);
+        step_perimeter<S, true>(p, (int)blockIdx.x, bc_row(p), macro);
     else
-        step_tile<S, CPT>(p, xcd_tile((int)blockIdx.x - np, p.n_main, p.rev), row, macro);
+        step_tile<S, CPT>(p, xcd_tile((int)blockIdx.x - np, p.n_main, p.rev), macro);
 }
 
 template <int S, int CPT>
@@ -1405,17 +1427,17 @@ __global__ __launch_bounds__(256) void k_batch_step(const Params *__restrict__ c
 {
     for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
         const int2 item = work[w];
+        touch_params(cases[2 * item.x + parity]);
         Params p = cases[2 * item.x + parity];
         p.k = k;
         p.base = base;
-        const int row = bc_row(p);
         if (item.y >= p.n_main) {
-            step_perimeter<S, !V4>(p, item.y - p.n_main, row, false);
+            step_perimeter<S, !V4>(p, item.y - p.n_main, bc_row(p), false);
         } else {
             if constexpr (V4)
-                step_tile_v4<S, CPT>(p, item.y, row, false);
+                step_tile_v4<S, CPT>(p, item.y, bc_row(p), false);
             else
-                step_tile<S, CPT>(p, item.y, row, false);
+                step_tile<S, CPT>(p, item.y, false);
         }
     }
 }
