@@ -1425,7 +1425,7 @@ __global__ __launch_bounds__(256) void k_step_v4(Params p)
     if ((int)blockIdx.x >= p.n_main)
         step_perimeter<S, false>(p, (int)blockIdx.x - p.n_main, row, macro);
     else
-        step_tile_v4<S, CPT>(p, xcd_tile(blockIdx.x, p.n_main, p.rev), row, macro);
+        step_tile_v4<S, CPT>(p, sweep_tile(blockIdx.x, p.n_main, p.rev), row, macro);
 }
 
 // Batched time step: many lattices in one launch.  cases[2*c + parity] holds
@@ -2012,10 +2012,13 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     h->cpt = d->cells_per_thread;
     if (d->storage == LBM_STORE_Q16) q16_maps(d->q16_range_host, h->qs, h->qo, h->qi, h->qc);
     // default nodes per work-item, the fastest on the 8192x2048 obstacle array:
-    // f16e and f16e8 1 (f16e8 at 8 waves per SIMD: 163.8 us against 166.6 at
-    // 2), the other forms 2 (FP16 122.0 us against 124.7 at 4, Q16 129.8
-    // against 139.2: at 2 nodes the 16-bit tiles fit 64 VGPRs, 8 waves/SIMD)
-    if (h->cpt == 0) h->cpt = shifted(d->storage) ? 1 : 2;
+    // FP16, BF16 and Q16 2 (FP16 109.4 us against 114.4 at 4 and 120.5 at 1,
+    // Q16 116.7 against 133.7 at 4: at 2 nodes the 16-bit tiles fit 64 VGPRs,
+    // 8 waves per SIMD), FP64, FP32, f16e and f16e8 1 (FP64 433 against 452
+    // at 2, FP32 220 against 226.5, f16e8 149.6 against 149.5)
+    if (h->cpt == 0)
+        h->cpt = (d->storage == LBM_STORE_F16 || d->storage == LBM_STORE_BF16 ||
+                  d->storage == LBM_STORE_Q16) ? 2 : 1;
     if (h->cpt != 1 && h->cpt != 2 && h->cpt != 4 && h->cpt != 8)
         return set_error(LBM_ERR_ARG, "cells_per_thread must be 1, 2, 4 or 8");
     // 16-bit forms: four nodes per work-item on request, when the lattice
