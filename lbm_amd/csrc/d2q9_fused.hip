@@ -1269,7 +1269,6 @@ __device__ __forceinline__ void touch_params(const Params &p)
 template <int S, int CPT>
 __device__ __forceinline__ void step_launch(const Params &p)
 {
-    touch_params(p);
     const bool macro = p.rho_out != nullptr;
     const int np = perim_groups(p.n_perim);
     if ((int)blockIdx.x < np)
