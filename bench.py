@@ -583,7 +583,7 @@ def summarize_profile(pdir, nodes, copy_bytes=2 << 30):
         return out
 
     def form_of(name):
-        m = re.search(r"k_step(_v4|_w7)?<(\d), (\d)>", name)
+        m = re.search(r"k_step(_v4|_w7|_w8)?<(\d), (\d)>", name)
         return (FORM_OF_STORE[m.group(2)], m.group(1) == "_v4", int(m.group(3))) if m else None
 
     stats = {r["Name"]: r for r in rows("trace", "kernel_stats.csv")}
