@@ -11,7 +11,7 @@ Tolerances, about three times the largest error measured on these two cases (vel
   BF16:  |u - u64| <= 2e-2 u_ref,    |rho - rho64| <= 2e-3,    |Cd - Cd64| <= 8e-3 Cd_ref
   F16E:  |u - u64| <= 1.5e-3 u_ref,  |rho - rho64| <= 1.5e-4,  |Cd - Cd64| <= 1e-3 Cd_ref
   Q16:   |u - u64| <= 8e-4 u_ref,    |rho - rho64| <= 8e-5,    |Cd - Cd64| <= 8e-4 Cd_ref
-  F16E8: |u - u64| <= 1.2e-3 u_ref,  |rho - rho64| <= 2.2e-4,  |Cd - Cd64| <= 1e-3 Cd_ref
+  F16E8: |u - u64| <= 1.2e-3 u_ref,  |rho - rho64| <= 1.5e-4,  |Cd - Cd64| <= 1e-3 Cd_ref
 (60 iterations of an impulsively started flow, the hardest regime for the
 reduced forms; long-run statistics are compared by bench.py)."""
 import numpy as np
@@ -22,7 +22,7 @@ from tests.cases import run_gpu, run_oracle, small_cavity, small_channel
 pytestmark = pytest.mark.gpu
 
 TOL = {"f32": (3e-6, 1e-6, 3e-6), "f16": (3e-3, 2e-4, 2e-3), "bf16": (2e-2, 2e-3, 8e-3),
-       "f16e": (1.5e-3, 1.5e-4, 1e-3), "q16": (8e-4, 8e-5, 8e-4), "f16e8": (1.2e-3, 2.2e-4, 1e-3)}
+       "f16e": (1.5e-3, 1.5e-4, 1e-3), "q16": (8e-4, 8e-5, 8e-4), "f16e8": (1.2e-3, 1.5e-4, 1e-3)}
 
 
 @pytest.mark.parametrize("form", ["f32", "f16", "bf16", "f16e", "q16", "f16e8"])
