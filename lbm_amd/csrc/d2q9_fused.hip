@@ -85,7 +85,7 @@ struct Params {
     const int *link_cells; // DRAG: distinct link nodes
     int n_link_cells;
     int tile_shift;        // log2(tile_cells)
-    int rev;               // sweep the XCD slabs last tile first (odd iterations)
+    int rev;               // sweep the tiles last first (odd iterations, large lattices)
     // Q16: h_q = qo[q] + qs[q] * i (decode); i = rint(h_q * qi[q] + qc[q]),
     // clamped to [-32767, 32767] (encode); qi = 1/qs, qc = -qo/qs.  qtab holds
     // the same four rows in device memory for directions known only at run
@@ -823,7 +823,8 @@ __device__ __forceinline__ void collide_store_h(const Params &p, int cell, const
 
 // Work-group index -> tile index, so that each XCD (work-groups are dealt
 // round-robin over the 8 XCDs) sweeps one contiguous slab of the lattice and
-// the rows its tiles share stay in that XCD's L2.
+// the rows its tiles share stay in that XCD's L2 (the initial and final
+// passes; the step sweeps tiles in dispatch order, sweep_tile).
 // With rev set the slab is swept last tile first.
 __device__ __forceinline__ int xcd_tile(int b, int nb, int rev = 0)
 {
@@ -1231,16 +1232,16 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, bool macro)
 // then fixes up obstacle-link nodes (tiles with links only) and collides.
 // Work-groups [n_main, grid) apply the Zou-He perimeter plan node by node.
 // Work-groups of the one-node step: first the perimeter work-groups (rounded
-// up to a multiple of 8 so that the tiles keep their XCD mapping), so that
-// their node-by-node chains run beside the tiles instead of after them
+// up to a multiple of 8, so that tile t runs on the XCD of work-group t), so
+// that their node-by-node chains run beside the tiles instead of after them
 // (FP16 on the 8192x2048 obstacle array: 127.9 -> 125.2 us), then the tiles.
 // Work-group index -> tile of the one-node step: consecutive tiles in
 // dispatch order (consecutive work-groups go to different XCDs), the last
-// tile first on odd iterations so that each sweep starts on the data the
-// previous one wrote last.  Against one contiguous slab per XCD (xcd_tile):
-// the eight XCDs then stream neighbouring regions of each plane at any time
-// (a data-movement-only kernel of the FP16 pattern on the 8192x2048 lattice:
-// 107.4 against 117.5 us).
+// tile first on odd iterations when rev is set, so that each sweep starts on
+// the data the previous one wrote last.  Against one contiguous slab per XCD
+// (xcd_tile): the eight XCDs then stream neighbouring regions of each plane
+// at any time (a data-movement-only kernel of the FP16 pattern on the
+// 8192x2048 lattice: 107.4 against 117.5 us).
 __device__ __forceinline__ int sweep_tile(int b, int nb, int rev)
 {
     return rev ? nb - 1 - b : b;
