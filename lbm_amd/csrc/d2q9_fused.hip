@@ -77,6 +77,8 @@ struct Params {
     const int *tile_slot;  // [n_tiles] -1 or slot of the tile's link offsets
     const int *cell_off;   // [n_slots][tile_cells+1]
     const uint32_t *cell_mask; // [n_slots][tile_cells/32] bit set = node has links
+    const int *wave_rng;   // float forms: [n_tiles][rng_stride] link-record bounds of
+    int rng_stride;        // the tile's 64-node runs (every tile; equal bounds: no links)
     int n_main;            // work-groups of interior tiles (the rest do the perimeter)
     int n_perim;           // perimeter nodes
     const LinkRec *links;  // sorted by node
@@ -1150,17 +1152,17 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, bool macro)
     // after the pull loads are issued: their dependent scalar loads then
     // overlap the pulls' latency instead of delaying them
     __builtin_amdgcn_sched_barrier(0);
-    const int slot = p.tile_slot ? ((const ci32 *)p.tile_slot)[tile] : -1;
+    const int slot = (S == LBM_STORE_F64 && p.tile_slot) ? ((const ci32 *)p.tile_slot)[tile] : -1;
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
         rbeg[c] = rend[c] = 0;
-        if (S != LBM_STORE_F64 && slot >= 0) {
-            // nodes [first, first + 64) of the wave: their records are
-            // [cell_off[first], cell_off[first + 64]) of the tile's table
-            const int lf = __builtin_amdgcn_readfirstlane((int)((threadIdx.x & ~63u) + c * 256));
-            const ci32 *co = (const ci32 *)p.cell_off + (size_t)slot * (256 * CPT + 1) + lf;
-            rbeg[c] = co[0];
-            rend[c] = co[64];
+        if (S != LBM_STORE_F64 && p.wave_rng) {
+            // run j = c*4 + wave of the tile (nodes [64 j, 64 j + 64)): its
+            // records are [wave_rng[j], wave_rng[j + 1]) of the tile's row
+            const int j = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) + c * 4);
+            const ci32 *wr = (const ci32 *)p.wave_rng + (size_t)tile * p.rng_stride + j;
+            rbeg[c] = wr[0];
+            rend[c] = wr[1];
         }
     }
     const int row = bc_row(p);
@@ -1597,6 +1599,7 @@ struct lbm_fused {
     void *ckpt[2] = {nullptr, nullptr};   // the two most recent checkpoints
     int64_t ckpt_it[2] = {-1, -1};
     int *d_tile_slot = nullptr, *d_cell_off = nullptr, *d_link_cells = nullptr;
+    int *d_wave_rng = nullptr, rng_stride = 0;
     uint32_t *d_cell_mask = nullptr;
     int n_perim = 0, n_perim_wg = 0;
     bool vec4 = false;
@@ -1665,6 +1668,8 @@ Params base_params(const lbm_fused *h)
     p.tile_slot = h->n_slots ? h->d_tile_slot : nullptr;
     p.cell_off = h->d_cell_off;
     p.cell_mask = h->d_cell_mask;
+    p.wave_rng = h->d_wave_rng;
+    p.rng_stride = h->rng_stride;
     p.n_main = h->n_tiles;
     p.n_perim = h->n_perim_wg;
     p.links = h->d_links;
@@ -1903,6 +1908,23 @@ int build_links(lbm_fused *h, const lbm_fused_desc *d)
     if (!cell_off.empty())
         LBM_HIP_TRY(hipMemcpy(h->d_cell_off, cell_off.data(), sizeof(int) * cell_off.size(),
                               hipMemcpyHostToDevice));
+    if (slot > 0 && h->storage != LBM_STORE_F64) {
+        // the record bounds of every 64-node run, per tile, padded to a power
+        // of two: one scalar load pair per wave, no dependent slot lookup
+        const int runs = tc / 64;
+        int rs = 1;
+        while (rs < runs + 1) rs <<= 1;
+        h->rng_stride = rs;
+        std::vector<int> rng((size_t)h->n_tiles * rs, 0);
+        for (int t = 0; t < h->n_tiles; ++t) {
+            if (tile_slot[t] < 0) continue;
+            for (int j = 0; j <= runs; ++j)
+                rng[(size_t)t * rs + j] = cell_off[(size_t)tile_slot[t] * (tc + 1) + j * 64];
+        }
+        if ((e = dmalloc(h, &h->d_wave_rng, sizeof(int) * rng.size()))) return e;
+        LBM_HIP_TRY(hipMemcpy(h->d_wave_rng, rng.data(), sizeof(int) * rng.size(),
+                              hipMemcpyHostToDevice));
+    }
     if (nl) {
         LBM_HIP_TRY(hipMemcpy(h->d_links, recs.data(), sizeof(LinkRec) * nl, hipMemcpyHostToDevice));
         LBM_HIP_TRY(hipMemcpy(h->d_plan_q, plan_q.data(), nl, hipMemcpyHostToDevice));
@@ -1929,6 +1951,7 @@ void free_all(lbm_fused *h)
     for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
     h->graphs.clear();
     void *ptrs[] = {h->buf[0], h->buf[1], h->ckpt[0], h->ckpt[1], h->d_tile_slot, h->d_cell_off, h->d_cell_mask,
+                    h->d_wave_rng,
                     h->d_link_cells, h->d_links, h->d_obs_off, h->d_plan_q, h->d_drag_ring,
                     h->d_drag_out, h->d_pool, h->d_inlet_off, h->d_base, h->d_zero, h->d_part,
                     h->d_qtab, h->d_amp_own, h->d_prof_own[0], h->d_prof_own[1],
