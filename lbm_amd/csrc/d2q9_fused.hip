@@ -79,6 +79,7 @@ struct Params {
     const uint32_t *cell_mask; // [n_slots][tile_cells/32] bit set = node has links
     const int *wave_rng;   // float forms: [n_tiles][rng_stride] link-record bounds of
     int rng_stride;        // the tile's 64-node runs (every tile; equal bounds: no links)
+    const uint32_t *run_bits; // float forms: bit r set = 64-node run r (node >> 6) has links
     int n_main;            // work-groups of interior tiles (the rest do the perimeter)
     int n_perim;           // perimeter nodes
     const LinkRec *links;  // sorted by node
@@ -1149,23 +1150,40 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, bool macro)
         }
     }
     // the link tables and the iteration's ring row (a device counter) only
-    // after the pull loads are issued: their dependent scalar loads then
-    // overlap the pulls' latency instead of delaying them
+    // after the pull loads are issued: their scalar loads then overlap the
+    // pulls' latency instead of delaying them.  The kernel arguments of this
+    // phase come in one batch, then the tile's word of run bits and the ring
+    // counter together: two rounds of scalar loads, and a third only for the
+    // runs with links (their record bounds)
     __builtin_amdgcn_sched_barrier(0);
     const int slot = (S == LBM_STORE_F64 && p.tile_slot) ? ((const ci32 *)p.tile_slot)[tile] : -1;
+    const int *wrp = p.wave_rng;
+    const uint32_t *rbp = p.run_bits;
+    const int *basep = p.base;
+    const int ring = p.ring, kk = p.k, rstride = p.rng_stride;
+    asm volatile("" ::"s"(wrp), "s"(rbp), "s"(basep), "s"(ring), "s"(kk), "s"(rstride));
+    // the tile's runs [4 CPT tile, 4 CPT tile + 4 CPT) lie in one word
+    const uint32_t r0 = (uint32_t)tile * (4 * CPT);
+    uint32_t word = 0;
+    if (S != LBM_STORE_F64 && wrp)
+        word = ((const __attribute__((address_space(4))) uint32_t *)rbp)[r0 >> 5];
+    int row = *(const ci32 *)basep;
+    asm volatile("" : "+s"(word), "+s"(row));
+    row = (row + kk) % ring;
+    row = row < 0 ? row + ring : row;
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
         rbeg[c] = rend[c] = 0;
-        if (S != LBM_STORE_F64 && p.wave_rng) {
-            // run j = c*4 + wave of the tile (nodes [64 j, 64 j + 64)): its
-            // records are [wave_rng[j], wave_rng[j + 1]) of the tile's row
-            const int j = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) + c * 4);
-            const ci32 *wr = (const ci32 *)p.wave_rng + (size_t)tile * p.rng_stride + j;
+        // run j = c*4 + wave of the tile (nodes [64 j, 64 j + 64)): for a
+        // run with links, its records [wave_rng[j], wave_rng[j + 1]) of the
+        // tile's row
+        const int j = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) + c * 4);
+        if (S != LBM_STORE_F64 && ((word >> ((r0 & 31) + j)) & 1u)) {
+            const ci32 *wr = (const ci32 *)wrp + (size_t)tile * rstride + j;
             rbeg[c] = wr[0];
             rend[c] = wr[1];
         }
     }
-    const int row = bc_row(p);
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
         // float forms: the run's links, link-major, before the nodes branch
@@ -1600,6 +1618,7 @@ struct lbm_fused {
     int64_t ckpt_it[2] = {-1, -1};
     int *d_tile_slot = nullptr, *d_cell_off = nullptr, *d_link_cells = nullptr;
     int *d_wave_rng = nullptr, rng_stride = 0;
+    uint32_t *d_run_bits = nullptr;
     uint32_t *d_cell_mask = nullptr;
     int n_perim = 0, n_perim_wg = 0;
     bool vec4 = false;
@@ -1670,6 +1689,7 @@ Params base_params(const lbm_fused *h)
     p.cell_mask = h->d_cell_mask;
     p.wave_rng = h->d_wave_rng;
     p.rng_stride = h->rng_stride;
+    p.run_bits = h->d_run_bits;
     p.n_main = h->n_tiles;
     p.n_perim = h->n_perim_wg;
     p.links = h->d_links;
@@ -1921,8 +1941,20 @@ int build_links(lbm_fused *h, const lbm_fused_desc *d)
             for (int j = 0; j <= runs; ++j)
                 rng[(size_t)t * rs + j] = cell_off[(size_t)tile_slot[t] * (tc + 1) + j * 64];
         }
+        // one bit per run: the table a wave reads first (4 KiB per 2^20 nodes,
+        // held in the L2), so that only the runs with links read their row
+        std::vector<uint32_t> bits(((size_t)h->n_tiles * runs + 31) / 32, 0u);
+        for (int t = 0; t < h->n_tiles; ++t)
+            for (int j = 0; j < runs; ++j) {
+                const size_t r = (size_t)t * runs + j;
+                if (rng[(size_t)t * rs + j + 1] > rng[(size_t)t * rs + j])
+                    bits[r >> 5] |= 1u << (r & 31);
+            }
         if ((e = dmalloc(h, &h->d_wave_rng, sizeof(int) * rng.size()))) return e;
+        if ((e = dmalloc(h, &h->d_run_bits, sizeof(uint32_t) * bits.size()))) return e;
         LBM_HIP_TRY(hipMemcpy(h->d_wave_rng, rng.data(), sizeof(int) * rng.size(),
+                              hipMemcpyHostToDevice));
+        LBM_HIP_TRY(hipMemcpy(h->d_run_bits, bits.data(), sizeof(uint32_t) * bits.size(),
                               hipMemcpyHostToDevice));
     }
     if (nl) {
@@ -1951,7 +1983,7 @@ void free_all(lbm_fused *h)
     for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
     h->graphs.clear();
     void *ptrs[] = {h->buf[0], h->buf[1], h->ckpt[0], h->ckpt[1], h->d_tile_slot, h->d_cell_off, h->d_cell_mask,
-                    h->d_wave_rng,
+                    h->d_wave_rng, h->d_run_bits,
                     h->d_link_cells, h->d_links, h->d_obs_off, h->d_plan_q, h->d_drag_ring,
                     h->d_drag_out, h->d_pool, h->d_inlet_off, h->d_base, h->d_zero, h->d_part,
                     h->d_qtab, h->d_amp_own, h->d_prof_own[0], h->d_prof_own[1],
