@@ -89,6 +89,7 @@ struct Params {
     int n_link_cells;
     int tile_shift;        // log2(tile_cells)
     int rev;               // sweep the tiles last first (odd iterations, large lattices)
+    int blk_shift;         // log2 of the tiles per XCD block (0: dispatch order)
     // Q16: h_q = qo[q] + qs[q] * i (decode); i = rint(h_q * qi[q] + qc[q]),
     // clamped to [-32767, 32767] (encode); qi = 1/qs, qc = -qo/qs.  qtab holds
     // the same four rows in device memory for directions known only at run
@@ -1261,10 +1262,22 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, bool macro)
 // the data the previous one wrote last.  Against one contiguous slab per XCD
 // (xcd_tile): the eight XCDs then stream neighbouring regions of each plane
 // at any time (a data-movement-only kernel of the FP16 pattern on the
-// 8192x2048 lattice: 107.4 against 117.5 us).
-__device__ __forceinline__ int sweep_tile(int b, int nb, int rev)
+// 8192x2048 lattice: 107.4 against 117.5 us).  With bs > 0, blocks of 2^bs
+// consecutive tiles (the float forms: one lattice column) go to one XCD
+// each, block j to XCD j % 8, and the XCDs still sweep neighbouring blocks.
+__device__ __forceinline__ int sweep_tile(int b, int nb, int rev, int bs = 0)
 {
-    return rev ? nb - 1 - b : b;
+    int t = b;
+    if (bs) {
+        // block (i >> bs) * 8 + xcd, tile i mod 2^bs of it; the tail of
+        // fewer than 8 blocks in dispatch order
+        const int span = 8 << bs;
+        if (b < nb - nb % span) {
+            const int xcd = b & 7, i = b >> 3;
+            t = ((((i >> bs) << 3) + xcd) << bs) + (i & ((1 << bs) - 1));
+        }
+    }
+    return rev ? nb - 1 - t : t;
 }
 
 __host__ __device__ constexpr int perim_groups(int n_perim)
@@ -1297,7 +1310,7 @@ __device__ __forceinline__ void step_launch(const Params &p)
     if ((int)blockIdx.x < np)
         step_perimeter<S, true>(p, (int)blockIdx.x, bc_row(p), macro);
     else
-        step_tile<S, CPT>(p, sweep_tile((int)blockIdx.x - np, p.n_main, p.rev), macro);
+        step_tile<S, CPT>(p, sweep_tile((int)blockIdx.x - np, p.n_main, p.rev, p.blk_shift), macro);
 }
 
 template <int S, int CPT>
@@ -1621,6 +1634,7 @@ struct lbm_fused {
     uint32_t *d_run_bits = nullptr;
     uint32_t *d_cell_mask = nullptr;
     int n_perim = 0, n_perim_wg = 0;
+    int blk_shift = 0;
     bool vec4 = false;
     const double *amp = nullptr;   // separable inlets (batched cases, lbm_fused_set_profiles)
     int amp_stride = 0;
@@ -1690,6 +1704,7 @@ Params base_params(const lbm_fused *h)
     p.wave_rng = h->d_wave_rng;
     p.rng_stride = h->rng_stride;
     p.run_bits = h->d_run_bits;
+    p.blk_shift = h->blk_shift;
     p.n_main = h->n_tiles;
     p.n_perim = h->n_perim_wg;
     p.links = h->d_links;
@@ -2101,6 +2116,16 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     h->tile_shift = 8 + (per_thread == 1 ? 0 : per_thread == 2 ? 1 : per_thread == 4 ? 2
                          : per_thread == 8 ? 3 : 4);
     h->n_tiles = (h->N + h->tile_cells - 1) / h->tile_cells;
+    // float forms: the tiles of one column on one XCD, the columns dealt
+    // round-robin over the XCDs (a power-of-two number of tiles per column;
+    // on the 8192x2048 obstacle array FP16 106.6 -> 105.5 us, f16e8 142.5 ->
+    // 139.2, f16e 157.2 -> 152.0; FP64 430.8 -> 450.5, so FP64 keeps the
+    // dispatch order)
+    if (d->storage != LBM_STORE_F64 && d->ny % h->tile_cells == 0) {
+        const int tpc = d->ny / h->tile_cells;
+        if ((tpc & (tpc - 1)) == 0)
+            while ((1 << h->blk_shift) < tpc) ++h->blk_shift;
+    }
     h->n_perim = 2 * d->nx + 2 * d->ny - 4;
     // nodes of the perimeter work-groups: the left and right columns (corners
     // included) where the one-node tiles complete their bottom and top wall
