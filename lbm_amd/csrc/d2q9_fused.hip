@@ -505,11 +505,11 @@ __device__ __forceinline__ void pull_checked(const Params &p, int x, int y,
 // the rest state bounces back exactly and float-rounded factors cost a
 // relative 1e-7 of the deviation, not of w.
 template <int S>
-__device__ __forceinline__ float link_h(const Params &p, int cell, const LinkRec &L)
+__device__ __forceinline__ float link_h(const Params &p, int cell, const LinkRec &L, float &A)
 {
     using St = Store<S>;
     const auto *src = static_cast<const typename St::s_t *>(p.src);
-    const float A = St::ldh(p, src, L.q, cell);
+    A = St::ldh(p, src, L.q, cell);
     if (!L.mode) return A;
     const float B = St::ldh(p, src, L.planeB, L.cellB);
     const float C = St::ldh(p, src, L.planeC, L.cellC);
@@ -537,7 +537,8 @@ __device__ __forceinline__ void apply_links(const Params &p, int cell, int lo, i
                 v = (T(L.c0) * A + T(L.c1) * B) + T(L.c2) * C;
             }
         } else {
-            v = weight<float>(L.qb) + link_h<S>(p, cell, L);
+            float A;
+            v = weight<float>(L.qb) + link_h<S>(p, cell, L, A);
         }
         // f[L.qb] = v with a register-resident array
 #pragma unroll
@@ -553,7 +554,8 @@ __device__ __forceinline__ void apply_links_h(const Params &p, int cell, int lo,
 {
     for (int k = lo; k < hi; ++k) {
         const LinkRec L = p.links[k];
-        const float v = link_h<S>(p, cell, L);
+        float A;
+        const float v = link_h<S>(p, cell, L, A);
 #pragma unroll
         for (int q = 1; q < 9; ++q)
             if (q == L.qb) h[q] = v;
@@ -1047,8 +1049,6 @@ template <int S>
 __device__ __forceinline__ void link_pass(const Params &p, int r0, int r1, int first, int row,
                                           float *lv, uint32_t *lf)
 {
-    using St = Store<S>;
-    const auto *src = static_cast<const typename St::s_t *>(p.src);
     const int lane = threadIdx.x & 63;
     const int lx = p.nx - 1, ly = p.ny - 1;
     lf[lane] = 0u;
@@ -1056,16 +1056,15 @@ __device__ __forceinline__ void link_pass(const Params &p, int r0, int r1, int f
         const int k = k0 + lane;
         if (k < r1) {
             const LinkRec L = p.links[k];
-            const float v = link_h<S>(p, L.cell, L);
+            float A;   // h_q of the link's node: the first source and the drag term's
+            const float v = link_h<S>(p, L.cell, L, A);
             const int o = L.cell - first;
             lv[o * 8 + (L.qb - 1)] = v;
             atomicOr(&lf[o], 1u << L.qb);
             const int x = node_x(p, L.cell), y = L.cell - x * p.ny;
-            if ((unsigned)(x - 1) < (unsigned)(lx - 1) && (unsigned)(y - 1) < (unsigned)(ly - 1)) {
-                const double hA = (double)St::ldh(p, src, L.q, L.cell);
+            if ((unsigned)(x - 1) < (unsigned)(lx - 1) && (unsigned)(y - 1) < (unsigned)(ly - 1))
                 p.drag_ring[(size_t)row * p.n_links + L.dslot] =
-                    2.0 * weight<double>(L.q) + (hA + (double)v);
-            }
+                    2.0 * weight<double>(L.q) + ((double)A + (double)v);
         }
     }
 }
