@@ -2646,7 +2646,7 @@ extern "C" int lbm_batch_create(const lbm_batch_desc *d, lbm_batch **out)
         return set_error(LBM_ERR_HIP, "batch amplitude table");
     }
     std::vector<Params> params;
-    std::vector<int2> work;
+    std::vector<int2> work, pwork;
     for (int c = 0; c < n; ++c) {
         const lbm_case_desc &cd = d->cases[c];
         lbm_fused_desc fd{};
@@ -2708,9 +2708,13 @@ extern "C" int lbm_batch_create(const lbm_batch_desc *d, lbm_batch **out)
             p.u_out = nullptr;
             params.push_back(p);
         }
-        const int blocks = h->n_tiles + (h->n_perim_wg + 255) / 256;
-        for (int k = 0; k < blocks; ++k) work.push_back(make_int2(c, k));
+        for (int k = 0; k < h->n_tiles; ++k) work.push_back(make_int2(c, k));
+        const int pblocks = (h->n_perim_wg + 255) / 256;
+        for (int k = 0; k < pblocks; ++k) pwork.push_back(make_int2(c, h->n_tiles + k));
     }
+    // every case's perimeter blocks first: their node-by-node chains then run
+    // beside the tiles instead of after them
+    work.insert(work.begin(), pwork.begin(), pwork.end());
     b->n_work = (int)work.size();
     if (hipMalloc((void **)&b->d_params, sizeof(Params) * params.size()) != hipSuccess ||
         hipMalloc((void **)&b->d_work, sizeof(int2) * work.size()) != hipSuccess ||
