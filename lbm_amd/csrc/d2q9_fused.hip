@@ -1331,7 +1331,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
 
 // And with 8 waves per SIMD (64 VGPRs): f16e8 with one node per work-item
 // fits without spilling (66 -> 64 VGPRs; 171.6 -> 163.8 us on the 8192x2048
-// obstacle array).
+// obstacle array); f16e with one spilled register (65 -> 64 VGPRs; 154.4 ->
+// 143.9 us).
 template <int S, int CPT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_step_w8(Params p)
 {
@@ -1752,7 +1753,7 @@ template <int S> void launch_step_s(const lbm_fused *h, const Params &p, hipStre
     const int grid = h->n_tiles + perim_groups(h->n_perim_wg);
     switch (h->cpt) {
     case 1:
-        if constexpr (S == LBM_STORE_F16E8) k_step_w8<S, 1><<<grid, 256, 0, s>>>(p);
+        if constexpr (S == LBM_STORE_F16E8 || S == LBM_STORE_F16E) k_step_w8<S, 1><<<grid, 256, 0, s>>>(p);
         else k_step<S, 1><<<grid, 256, 0, s>>>(p);
         break;
     case 2:
