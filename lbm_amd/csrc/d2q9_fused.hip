@@ -2716,6 +2716,22 @@ extern "C" int lbm_batch_create(const lbm_batch_desc *d, lbm_batch **out)
     // every case's perimeter blocks first: their node-by-node chains then run
     // beside the tiles instead of after them
     work.insert(work.begin(), pwork.begin(), pwork.end());
+    // then blocks of 16 consecutive items on one XCD: with one work-group per
+    // item, work-group g (on XCD g mod 8) takes item ((i >> 4) * 8 + g mod 8)
+    // * 16 + (i mod 16), i = g / 8; the tail of fewer than 8 blocks keeps its
+    // order (256-cavity ensemble:
+    // FP32 66.6 -> 72.0 GLUPS, FP64 36.6 -> 38.7; blocks of 4: 70.4, of 8:
+    // 71.3, of 32: 72.4 / 38.4, of 64: 70.9)
+    {
+        constexpr int bs = 4;
+        const int nw = (int)work.size(), full = nw - nw % (8 << bs);
+        std::vector<int2> order(work);
+        for (int g = 0; g < full; ++g) {
+            const int xcd = g & 7, i = g >> 3;
+            order[g] = work[((((i >> bs) << 3) + xcd) << bs) + (i & ((1 << bs) - 1))];
+        }
+        work.swap(order);
+    }
     b->n_work = (int)work.size();
     if (hipMalloc((void **)&b->d_params, sizeof(Params) * params.size()) != hipSuccess ||
         hipMalloc((void **)&b->d_work, sizeof(int2) * work.size()) != hipSuccess ||
