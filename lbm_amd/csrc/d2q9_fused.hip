@@ -2116,12 +2116,18 @@ extern "C" int lbm_fused_create(const lbm_fused_desc *d, lbm_fused **out)
     h->tile_shift = 8 + (per_thread == 1 ? 0 : per_thread == 2 ? 1 : per_thread == 4 ? 2
                          : per_thread == 8 ? 3 : 4);
     h->n_tiles = (h->N + h->tile_cells - 1) / h->tile_cells;
-    // float forms: the tiles of one column on one XCD, the columns dealt
-    // round-robin over the XCDs (a power-of-two number of tiles per column;
-    // on the 8192x2048 obstacle array FP16 106.6 -> 105.5 us, f16e8 142.5 ->
-    // 139.2, f16e 157.2 -> 152.0; FP64 430.8 -> 450.5, so FP64 keeps the
-    // dispatch order)
-    if (d->storage != LBM_STORE_F64 && d->ny % h->tile_cells == 0) {
+    // Tiles dealt to the XCDs in blocks of consecutive tiles.  Lattices of
+    // up to 128 MB of state (one sweep direction, each tile on the same XCD
+    // every iteration): blocks of 4, so that a tile's neighbours share its
+    // XCD's L2 (Turek 1073x200 FP64 through the drop-in run: 14.3 -> 18.2
+    // GLUPS).  Larger ones, float forms: the tiles of one column on one XCD,
+    // the columns round-robin over the XCDs (a power-of-two number of tiles
+    // per column; on the 8192x2048 obstacle array FP16 106.6 -> 105.5 us,
+    // f16e8 142.5 -> 139.2, f16e 157.2 -> 152.0; FP64 430.8 -> 450.5, so
+    // FP64 keeps the dispatch order there).
+    if (state_bytes(d->storage, h->N) <= ((size_t)128 << 20)) {
+        h->blk_shift = 2;
+    } else if (d->storage != LBM_STORE_F64 && d->ny % h->tile_cells == 0) {
         const int tpc = d->ny / h->tile_cells;
         if ((tpc & (tpc - 1)) == 0)
             while ((1 << h->blk_shift) < tpc) ++h->blk_shift;
