@@ -22,6 +22,18 @@ __host__ __device__ constexpr T weight(int q)
     return q == 0 ? T(4.0 / 9.0) : (q <= 4 ? T(1.0 / 9.0) : T(1.0 / 36.0));
 }
 
+// Work-group b of nb (work-groups are dealt round-robin over the 8 XCDs, so b
+// runs on XCD b mod 8) -> the unit of work it takes when units go to the XCDs
+// in blocks of 2^bs consecutive ones: block (i >> bs) * 8 + b mod 8, unit
+// i mod 2^bs of it, i = b / 8; the tail of fewer than 8 blocks keeps its
+// order.  bs = 0: b itself.
+__host__ __device__ constexpr int xcd_block_unit(int b, int nb, int bs)
+{
+    return (bs > 0 && b < nb - nb % (8 << bs))
+               ? (((((b >> 3) >> bs) << 3) + (b & 7)) << bs) + ((b >> 3) & ((1 << bs) - 1))
+               : b;
+}
+
 // Error reporting of the C ABI (per calling thread).
 int set_error(int code, const std::string &msg);
 

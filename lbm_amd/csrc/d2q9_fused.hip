@@ -1262,20 +1262,12 @@ __device__ __forceinline__ void step_tile(const Params &p, int tile, bool macro)
 // (xcd_tile): the eight XCDs then stream neighbouring regions of each plane
 // at any time (a data-movement-only kernel of the FP16 pattern on the
 // 8192x2048 lattice: 107.4 against 117.5 us).  With bs > 0, blocks of 2^bs
-// consecutive tiles (the float forms: one lattice column) go to one XCD
-// each, block j to XCD j % 8, and the XCDs still sweep neighbouring blocks.
+// consecutive tiles (4 on lattices up to 128 MB; one lattice column for the
+// float forms above) go to one XCD each, block j to XCD j % 8, and the XCDs
+// still sweep neighbouring blocks.
 __device__ __forceinline__ int sweep_tile(int b, int nb, int rev, int bs = 0)
 {
-    int t = b;
-    if (bs) {
-        // block (i >> bs) * 8 + xcd, tile i mod 2^bs of it; the tail of
-        // fewer than 8 blocks in dispatch order
-        const int span = 8 << bs;
-        if (b < nb - nb % span) {
-            const int xcd = b & 7, i = b >> 3;
-            t = ((((i >> bs) << 3) + xcd) << bs) + (i & ((1 << bs) - 1));
-        }
-    }
+    const int t = xcd_block_unit(b, nb, bs);
     return rev ? nb - 1 - t : t;
 }
 
@@ -2729,13 +2721,9 @@ extern "C" int lbm_batch_create(const lbm_batch_desc *d, lbm_batch **out)
     // FP32 66.6 -> 72.0 GLUPS, FP64 36.6 -> 38.7; blocks of 4: 70.4, of 8:
     // 71.3, of 32: 72.4 / 38.4, of 64: 70.9)
     {
-        constexpr int bs = 4;
-        const int nw = (int)work.size(), full = nw - nw % (8 << bs);
+        const int nw = (int)work.size();
         std::vector<int2> order(work);
-        for (int g = 0; g < full; ++g) {
-            const int xcd = g & 7, i = g >> 3;
-            order[g] = work[((((i >> bs) << 3) + xcd) << bs) + (i & ((1 << bs) - 1))];
-        }
+        for (int g = 0; g < nw; ++g) order[g] = work[xcd_block_unit(g, nw, 4)];
         work.swap(order);
     }
     b->n_work = (int)work.size();
