@@ -1715,16 +1715,19 @@ Params base_params(const lbm_fused *h)
     return p;
 }
 
-// Lattices whose state buffer exceeds 128 MB sweep the tiles in reverse on
+// Lattices whose state buffer holds 128-512 MB sweep the tiles in reverse on
 // odd iterations, so that they start on the nodes the previous iteration
 // wrote last, which may still sit in the 256 MB Infinity Cache (8192x2048
-// obstacle array: FP16 +5 %, FP64 +3 %).  Smaller lattices keep one sweep
-// direction: tile t then runs on work-group t, i.e. on the same XCD, every
-// iteration, and reads from that XCD's L2 what it wrote there the iteration
-// before (Turek 1073x200, FP64: 15.1 -> 13.1 us per iteration).
+// obstacle array, FP16: 111.2 -> 104.8 us).  Above 512 MB of state, where
+// the cache holds a small part of what was written last, one direction
+// again (FP32 604 MB: 214.0 -> 212.3 us, FP64 1.2 GB: 436.4 -> 427.5).
+// Smaller lattices keep one sweep direction: tile t then runs on the same XCD
+// every iteration and reads from that XCD's L2 what it wrote there the
+// iteration before (Turek 1073x200, FP64: 15.1 -> 13.1 us per iteration).
 int sweep_rev(const lbm_fused *h, int64_t it)
 {
-    return state_bytes(h->storage, h->N) > ((size_t)128 << 20) ? (int)(it & 1) : 0;
+    const size_t b = state_bytes(h->storage, h->N);
+    return (b > ((size_t)128 << 20) && b <= ((size_t)512 << 20)) ? (int)(it & 1) : 0;
 }
 
 int row_of(const lbm_fused *h, int64_t it)

@@ -163,8 +163,8 @@ def run_pair(make, iterations, chunk=16, **kw):
 
 
 def array_domain_case(storage, iterations=80, vector_width=0, cpt=0, ny=64, n_obs=3, seed=1,
-                      side=(6, 14)):
-    """Obstacle-array channel (nx = 4 ny) run directly through FusedDomain;
+                      side=(6, 14), aspect=4):
+    """Obstacle-array channel (nx = aspect ny) run directly through FusedDomain;
     returns the final fields (epilogue), the drag series and the domain shape.
     side: range of the squares' side lengths."""
     import math
@@ -176,7 +176,7 @@ def array_domain_case(storage, iterations=80, vector_width=0, cpt=0, ny=64, n_ob
     from lbm_amd.app.configs import obstacle_array
     from lbm_amd.domain import FusedDomain, pack_inlets
     from lbm_amd.shapes import generate_shape
-    app = obstacle_array(ny=ny, aspect=4, n_obs=n_obs, seed=seed, precision=storage,
+    app = obstacle_array(ny=ny, aspect=aspect, n_obs=n_obs, seed=seed, precision=storage,
                          iterations=iterations, sigma=20, n_spts=100, side_lo=side[0],
                          side_hi=side[1])
     lat = types.SimpleNamespace(nx=app.nx, ny=app.ny, x_min=app.x_min, x_max=app.x_max,
