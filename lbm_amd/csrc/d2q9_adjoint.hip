@@ -415,10 +415,13 @@ template <typename T, int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_pbb_adj(
     PbbParams p, int eb)
 {
+    // interior node blocks dealt to the XCDs four at a time (neighbouring
+    // rows in one L2; FP64 backward 64.2 -> 63.0 ms on the 1024x256 channel)
     if ((int)blockIdx.x < eb)
         adj_perimeter_item<T>(p, blockIdx.x * 256 + threadIdx.x);
     else
-        adj_interior_node<T>(p, ((int)blockIdx.x - eb) * 256 + threadIdx.x);
+        adj_interior_node<T>(
+            p, xcd_block_unit((int)blockIdx.x - eb, (int)gridDim.x - eb, 2) * 256 + threadIdx.x);
 }
 
 __global__ void k_pbb_sum_rows(const double *part, int nb, int rows, double *out)
